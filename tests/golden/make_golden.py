@@ -1,0 +1,125 @@
+"""Write the golden fixtures from the reference Fortran built by flang.
+
+Run in a container that holds the reference sources (default
+/root/reference, override with MFCFD_REFERENCE_DIR):
+
+    python tests/golden/make_golden.py [--c1]
+
+For every fixture the script builds a small cloud with the project's
+generator, compiles the reference for that cloud and those flow conditions
+(oracle/build_ref.sh: flang -O2, default x86-64 target, the reference's own
+sources read in place), runs its entry points through oracle/ref_driver.f90
+and stores inputs and outputs as .npz:
+
+  cloud      x, y, z, tan1, tan2, nor, status, nbhs, conn, min_dist
+  pre        split counts and lists, alias, point_with_alias
+  hist       res_new, residue, max_res per iteration
+  start/step the last iteration entry point by entry point
+  final      prim, U, flux_res, delUp, delUn after the last iteration
+
+The reference's implicit sweep visits point_with_alias(N) = 0 and writes
+delUp(:,0)/delUn(:,0) just below those arrays.  Above glibc's mmap threshold
+(every production-size cloud) the writes land in page slack; below it they
+would overwrite the tail of the previously allocated array.  The driver
+therefore runs with glibc.malloc.mmap_threshold=4096 so small clouds are laid
+out like large ones.
+
+--c1 additionally runs the 50k-point explicit configuration for 1000
+iterations (about 20 minutes on one core) and stores its residual history
+and final wall state.
+"""
+
+from __future__ import annotations
+
+import argparse
+import hashlib
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import importlib  # noqa: E402
+
+import orc  # noqa: E402
+
+pc = importlib.import_module("3d-mfcfd_amd").pointcloud
+
+FIXTURES = {
+    # name: (cloud kwargs, mode, mach, aoa, cfl, iterations)
+    "explicit_blocks": (dict(n_interior=600, axes=(0.5, 0.3, 0.2), seed=11, order="blocks"),
+                        "explicit", 0.6, 2.0, 0.2, 40),
+    "explicit_morton": (dict(n_interior=1200, axes=(0.5, 0.167, 0.05), seed=12, order="morton"),
+                        "explicit", 0.84, 3.06, 0.2, 30),
+    "implicit_morton": (dict(n_interior=1200, axes=(0.5, 0.167, 0.05), seed=11, order="morton"),
+                        "implicit", 0.84, 3.06, 2.0, 30),
+}
+
+ENV = dict(os.environ, GLIBC_TUNABLES="glibc.malloc.mmap_threshold=4096")
+
+
+def cloud_digest(cl) -> str:
+    h = hashlib.sha256()
+    for a in (cl.x, cl.y, cl.z, cl.tan1, cl.tan2, cl.nor, cl.status, cl.nbhs, cl.conn, cl.min_dist):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def make_fixture(name):
+    kw, mode, mach, aoa, cfl, iters = FIXTURES[name]
+    cl = pc.fixture_cloud(**kw)
+    exe = orc.build_reference(f"golden_{name}", cl.n, 1000, mach, aoa, cfl, mode)
+    prim0 = np.tile(pc.freestream_prim(mach, aoa), (cl.n, 1))
+    ref = orc.run_reference(exe, cl, prim0, iters, stepwise=True, env=ENV)
+    out = dict(x=cl.x, y=cl.y, z=cl.z, tan1=cl.tan1, tan2=cl.tan2, nor=cl.nor,
+               status=cl.status, nbhs=cl.nbhs, conn=cl.conn, min_dist=cl.min_dist,
+               prim0=prim0, mode=np.array(mode), mach=mach, aoa=aoa, cfl=cfl,
+               iterations=iters, digest=np.array(cloud_digest(cl)))
+    for key in ("split_counts", "split_lists", "alias", "point_with_alias", "hist",
+                "start_prim", "start_delUp", "start_delUn", "start_res_old",
+                "step_U", "step_q", "step_qm", "step_dq", "step_delt", "step_flux_res",
+                "step_delUp", "step_delUn", "step_prim", "step_delUp_after", "step_res_new",
+                "final_prim", "final_U", "final_flux_res", "final_delUp", "final_delUn"):
+        out[key] = ref[key]
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: N={cl.n} {cl.counts()} -> {path} ({os.path.getsize(path)} bytes)")
+
+
+def make_c1():
+    c = pc.CONFIGS["c1_50k"]
+    cl = pc.config_cloud("c1_50k")
+    exe = orc.build_reference("c1_50k", cl.n, c["iters"], c["mach"], c["aoa"], c["cfl"], "explicit")
+    prim0 = np.tile(pc.freestream_prim(c["mach"], c["aoa"]), (cl.n, 1))
+    t0 = time.time()
+    ref = orc.run_reference(exe, cl, prim0, c["iters"], stepwise=False, env=ENV)
+    wall = cl.status == pc.STATUS_WALL
+    path = os.path.join(HERE, "c1_50k_history.npz")
+    np.savez_compressed(path, hist=ref["hist"], digest=np.array(cloud_digest(cl)),
+                        wall_index=np.flatnonzero(wall).astype(np.int32),
+                        wall_prim=ref["final_prim"][wall], seconds=ref["seconds"],
+                        timed_iters=ref["timed_iters"], mach=c["mach"], aoa=c["aoa"],
+                        cfl=c["cfl"], iterations=c["iters"])
+    print(f"c1_50k: {time.time() - t0:.0f} s -> {path}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--c1", action="store_true", help="also run the 50k x 1000 history")
+    ap.add_argument("--only", default=None)
+    args = ap.parse_args()
+    for name in FIXTURES:
+        if args.only and name != args.only:
+            continue
+        make_fixture(name)
+    if args.c1:
+        make_c1()
+
+
+if __name__ == "__main__":
+    main()

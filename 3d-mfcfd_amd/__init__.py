@@ -13,5 +13,6 @@ registered on import.
 import sys as _sys
 
 from . import pointcloud  # noqa: F401
+from . import solver  # noqa: F401
 
 _sys.modules.setdefault("mfcfd_amd", _sys.modules[__name__])

@@ -1,0 +1,640 @@
+// capi.hip -- C ABI of libmfcfd_hip.so (declared in include/mfcfd_hip.h).
+//
+// A context owns the HBM copies of the point cloud and of the flow state in
+// the reference's layouts, the class index lists, the sweep schedule and
+// the residual buffers.  Entry points enqueue kernels on the context's
+// stream and return; only the fpi_solver/run/get_residual/download calls
+// wait for the device.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mfcfd_hip.h"
+#include "device_state.h"
+#include "kernels.h"
+#include "preprocess.h"
+
+using namespace mfcfd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(MFCFD_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                         \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct mfcfd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool have_params = false;
+    bool have_geom = false;
+    mfcfd_params params{};
+    FlowParams fp{};
+
+    int n = 0;
+    // geometry
+    DevBuf<double4> pos;
+    DevBuf<double> frame;
+    DevBuf<int> status, nbhs, conn, split_cnt, split, alias, respos;
+    DevBuf<int> idx_interior, idx_wall, idx_outer, idx_sup_in, idx_sup_out;
+    TypeLists tl{};
+    // state
+    DevBuf<double> prim, U, q, qm, dq, delt, flux_res, delUp, delUn, dinv, temp;
+    // residual
+    DevBuf<double> partial;
+    DevBuf<int> partial_pos;
+    int n_partial = 0;
+    DevBuf<ResidualRecord> rec;
+    DevBuf<double> hist;
+    ResidualRecord* host_rec = nullptr;  // pinned
+    // sweep schedule
+    Schedule sched;
+    DevBuf<int> fwd_points, bwd_points;
+    bool implicit_supported = true;
+    // bookkeeping of fpi_solver.f90:38-43
+    double res_old = 0.0, res_new = 0.0, residue = 0.0, max_res = 0.0, sum_res_sqr = 0.0;
+    int max_res_point = 0, last_t = 0;
+
+    DevGeom geom() const {
+        DevGeom g;
+        g.n = n;
+        g.pos = pos.p;
+        g.frame = frame.p;
+        g.status = status.p;
+        g.nbhs = nbhs.p;
+        g.conn = conn.p;
+        g.split_cnt = split_cnt.p;
+        g.split = split.p;
+        g.split_stride = 22LL * n + kSplitSlack;
+        g.alias = alias.p;
+        g.respos = respos.p;
+        return g;
+    }
+    DevState state() const {
+        DevState s;
+        s.prim = prim.p;
+        s.U = U.p;
+        s.q = q.p;
+        s.qm = qm.p;
+        s.dq = dq.p;
+        s.delt = delt.p;
+        s.flux_res = flux_res.p;
+        s.delUp = delUp.p;
+        s.delUn = delUn.p;
+        s.dinv = dinv.p;
+        return s;
+    }
+    ~mfcfd_ctx() {
+        DevBuf<double>* dbl[] = {&frame, &prim, &U, &q, &qm, &dq, &delt, &flux_res, &delUp, &delUn, &dinv,
+                                 &temp, &partial, &hist};
+        for (auto* b : dbl) b->release();
+        DevBuf<int>* ints[] = {&status, &nbhs, &conn, &split_cnt, &split, &alias, &respos, &idx_interior,
+                               &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &partial_pos,
+                               &fwd_points, &bwd_points};
+        for (auto* b : ints) b->release();
+        pos.release();
+        rec.release();
+        if (host_rec) (void)hipHostFree(host_rec);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int check_ctx(mfcfd_ctx* c, bool need_geom) {
+    if (!c) return fail(MFCFD_EINVAL, "null context");
+    if (!c->have_params) return fail(MFCFD_ESTATE, "mfcfd_set_params has not been called");
+    if (need_geom && !c->have_geom) return fail(MFCFD_ESTATE, "mfcfd_bind_geometry has not been called");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(MFCFD_EHIP, "hipSetDevice(%d) failed", c->device);
+    return MFCFD_OK;
+}
+
+template <typename T>
+int upload(DevBuf<T>& b, const T* host, size_t count, hipStream_t st) {
+    if (b.alloc(count) != hipSuccess) return fail(MFCFD_ENOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+    if (count) HIP_TRY(hipMemcpyAsync(b.p, host, count * sizeof(T), hipMemcpyHostToDevice, st));
+    return MFCFD_OK;
+}
+
+struct ArrayDesc {
+    DevBuf<double>* buf;
+    size_t per_point;
+};
+
+ArrayDesc array_of(mfcfd_ctx* c, int id) {
+    switch (id) {
+        case MFCFD_ARR_PRIM: return {&c->prim, 5};
+        case MFCFD_ARR_U: return {&c->U, 5};
+        case MFCFD_ARR_Q: return {&c->q, 5};
+        case MFCFD_ARR_QM: return {&c->qm, 10};
+        case MFCFD_ARR_DQ: return {&c->dq, 15};
+        case MFCFD_ARR_DELT: return {&c->delt, 1};
+        case MFCFD_ARR_FLUX_RES: return {&c->flux_res, 5};
+        case MFCFD_ARR_DELUP: return {&c->delUp, 5};
+        case MFCFD_ARR_DELUN: return {&c->delUn, 5};
+        default: return {nullptr, 0};
+    }
+}
+
+int launch_error(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(MFCFD_EHIP, "%s launch: %s", what, hipGetErrorString(e));
+    return MFCFD_OK;
+}
+
+int do_sweeps(mfcfd_ctx* c) {
+    const DevGeom g = c->geom();
+    const DevState s = c->state();
+    if (c->params.mode == MFCFD_EXPLICIT) {
+        launch_explicit_sweep(g, s, c->stream);
+        return launch_error("explicit sweep");
+    }
+    if (!c->implicit_supported)
+        return fail(MFCFD_EUNSUPPORTED, "implicit sweeps with supersonic points are not supported");
+    launch_dinv(g, s, c->fp, c->tl, c->stream);
+    SweepSchedule sch;
+    sch.n_levels_fwd = c->sched.n_levels_fwd;
+    sch.n_levels_bwd = c->sched.n_levels_bwd;
+    sch.fwd_ptr = c->sched.fwd_ptr.data();
+    sch.bwd_ptr = c->sched.bwd_ptr.data();
+    sch.fwd_points = c->fwd_points.p;
+    sch.bwd_points = c->bwd_points.p;
+    sch.n_swept = (int)c->sched.order.size();
+    launch_sweeps_levels(g, s, c->fp, sch, c->stream);
+    return launch_error("implicit sweeps");
+}
+
+int do_state_update(mfcfd_ctx* c, double* hist, int hist_index) {
+    launch_state_update(c->geom(), c->state(), c->fp, c->tl, c->partial.p, c->partial_pos.p, c->n_partial,
+                        c->rec.p, hist, hist_index, c->stream);
+    return launch_error("state update");
+}
+
+int do_iteration(mfcfd_ctx* c, double* hist, int hist_index) {
+    const DevGeom g = c->geom();
+    const DevState s = c->state();
+    launch_conserved_and_q(g, s, c->stream);
+    if (c->fp.inner_iterations > 0) {
+        launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
+        launch_timestep(g, s, c->fp, c->stream);
+    } else {
+        launch_qderiv_timestep(g, s, c->fp, c->stream);
+    }
+    launch_flux_residual(g, s, c->fp, c->tl, c->stream);
+    int rc = launch_error("iteration");
+    if (rc) return rc;
+    if ((rc = do_sweeps(c))) return rc;
+    return do_state_update(c, hist, hist_index);
+}
+
+// fpi_solver.f90:38-43
+void residue_bookkeeping(mfcfd_ctx* c, int t, double res_new) {
+    c->res_new = res_new;
+    if (t <= 2) {
+        c->res_old = res_new;
+        c->residue = 0.0;
+    } else {
+        c->residue = std::log10(res_new / c->res_old);
+    }
+    c->last_t = t;
+}
+
+int fetch_record(mfcfd_ctx* c) {
+    HIP_TRY(hipMemcpyAsync(c->host_rec, c->rec.p, sizeof(ResidualRecord), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->res_new = c->host_rec->res_new;
+    c->sum_res_sqr = c->host_rec->sum_res_sqr;
+    c->max_res = c->host_rec->max_res;
+    if (c->host_rec->max_res_point > 0) c->max_res_point = c->host_rec->max_res_point;
+    return MFCFD_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mfcfd_last_error(void) { return g_last_error.c_str(); }
+int mfcfd_abi_version(void) { return MFCFD_ABI_VERSION; }
+
+int mfcfd_create(int device, mfcfd_ctx** out) {
+    if (!out) return fail(MFCFD_EINVAL, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(MFCFD_EINVAL, "device %d not present (%d devices)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    mfcfd_ctx* c = new (std::nothrow) mfcfd_ctx();
+    if (!c) return fail(MFCFD_ENOMEM, "context allocation failed");
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(MFCFD_EHIP, "hipStreamCreate failed");
+    }
+    if (c->rec.alloc(1) != hipSuccess ||
+        hipHostMalloc((void**)&c->host_rec, sizeof(ResidualRecord), hipHostMallocDefault) != hipSuccess) {
+        delete c;
+        return fail(MFCFD_ENOMEM, "residual buffers");
+    }
+    *out = c;
+    return MFCFD_OK;
+}
+
+int mfcfd_destroy(mfcfd_ctx* ctx) {
+    if (!ctx) return MFCFD_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return MFCFD_OK;
+}
+
+int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
+    if (!c || !p) return fail(MFCFD_EINVAL, "null argument");
+    if (p->mode != MFCFD_EXPLICIT && p->mode != MFCFD_IMPLICIT) return fail(MFCFD_EINVAL, "mode %d", p->mode);
+    if (p->precision != MFCFD_FP64 && p->precision != MFCFD_MIXED)
+        return fail(MFCFD_EINVAL, "precision %d", p->precision);
+    if (p->precision == MFCFD_MIXED) return fail(MFCFD_EUNSUPPORTED, "mixed precision is not built in");
+    if (!(p->cfl > 0.0)) return fail(MFCFD_EINVAL, "CFL must be positive");
+    if (p->inner_iterations < 0) return fail(MFCFD_EINVAL, "inner_iterations < 0");
+    c->params = *p;
+    // parameter.f90:13-33
+    FlowParams& f = c->fp;
+    f.cfl = p->cfl;
+    f.vl_const = p->vl_const;
+    f.power = p->power;
+    f.pi = 4.0 * std::atan(1.0);
+    const double theta = p->aoa * f.pi / 180.0;
+    f.rho_inf = 1.0;
+    f.u1_inf = p->mach * std::cos(theta);
+    f.u2_inf = p->mach * std::sin(theta);
+    f.u3_inf = 0.0;
+    f.pr_inf = 1.0 / 1.4;
+    f.inner_iterations = p->inner_iterations;
+    c->have_params = true;
+    if (c->have_geom && p->inner_iterations > 0 && c->temp.n == 0) {
+        if (c->temp.alloc(15ULL * c->n) != hipSuccess) return fail(MFCFD_ENOMEM, "temp(3,5,N)");
+    }
+    return MFCFD_OK;
+}
+
+int mfcfd_generate_split_stencils(int n, const double* x, const double* y, const double* z,
+                                  const double* tan1, const double* tan2, const double* nor,
+                                  const int* status, const int* nbhs, const int* conn, int* split_nbhs[6],
+                                  int* split_conn[6]) {
+    if (n <= 0 || !x || !y || !z || !tan1 || !tan2 || !nor || !status || !nbhs || !conn)
+        return fail(MFCFD_EINVAL, "null or empty input");
+    for (int d = 0; d < 6; ++d)
+        if (!split_nbhs[d] || !split_conn[d]) return fail(MFCFD_EINVAL, "null split array %d", d);
+    for (int i = 0; i < n; ++i) {
+        if (nbhs[i] < 0 || nbhs[i] > kConnCols) return fail(MFCFD_EINVAL, "point %d: nbhs=%d", i + 1, nbhs[i]);
+        for (int j = 0; j < nbhs[i]; ++j)
+            if (conn[27LL * i + j] < 1 || conn[27LL * i + j] > n)
+                return fail(MFCFD_EINVAL, "point %d: conn(%d)=%d out of range", i + 1, j + 1, conn[27LL * i + j]);
+    }
+    generate_split_stencils(n, x, y, z, tan1, tan2, nor, status, nbhs, conn, split_nbhs, split_conn);
+    return MFCFD_OK;
+}
+
+int mfcfd_aliasing(int n, const int* nbhs, const int* conn, int* alias, int* point_with_alias) {
+    if (n <= 0 || !nbhs || !conn || !alias || !point_with_alias) return fail(MFCFD_EINVAL, "null or empty input");
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < nbhs[i]; ++j)
+            if (conn[27LL * i + j] < 1 || conn[27LL * i + j] > n)
+                return fail(MFCFD_EINVAL, "point %d: conn(%d) out of range", i + 1, j + 1);
+    aliasing(n, nbhs, conn, alias, point_with_alias);
+    return MFCFD_OK;
+}
+
+int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if (!gm || gm->n <= 0) return fail(MFCFD_EINVAL, "empty geometry");
+    const int n = gm->n;
+    const double* dp[] = {gm->x, gm->y, gm->z, gm->tan1, gm->tan2, gm->nor, gm->min_dist};
+    for (auto* p : dp)
+        if (!p) return fail(MFCFD_EINVAL, "null geometry array");
+    if (!gm->status || !gm->nbhs || !gm->conn || !gm->alias || !gm->point_with_alias)
+        return fail(MFCFD_EINVAL, "null index array");
+    for (int d = 0; d < 6; ++d)
+        if (!gm->split_nbhs[d] || !gm->split_conn[d]) return fail(MFCFD_EINVAL, "null split array %d", d);
+
+    // validate: neighbour numbers, list lengths and the last point's spill
+    std::vector<int> cls_count(7, 0);
+    for (int i = 0; i < n; ++i) {
+        const int st = gm->status[i];
+        if (st != 0 && st != 1 && st != 2 && st != 5 && st != 6)
+            return fail(MFCFD_EINVAL, "point %d: unknown status %d", i + 1, st);
+        cls_count[st]++;
+        if (gm->nbhs[i] < 0 || gm->nbhs[i] > kConnCols)
+            return fail(MFCFD_EINVAL, "point %d: nbhs=%d outside 0..27", i + 1, gm->nbhs[i]);
+        for (int j = 0; j < gm->nbhs[i]; ++j) {
+            const int k = gm->conn[27LL * i + j];
+            if (k < 1 || k > n) return fail(MFCFD_EINVAL, "point %d: conn(%d)=%d out of range", i + 1, j + 1, k);
+        }
+    }
+    for (int d = 0; d < 6; ++d) {
+        for (int i = 0; i < n; ++i) {
+            const int cnt = gm->split_nbhs[d][i];
+            if (cnt < 0 || cnt > kConnCols)
+                return fail(MFCFD_EINVAL, "point %d: split list %d has %d entries", i + 1, d, cnt);
+        }
+        if (gm->split_nbhs[d][n - 1] > kSplitCols && !(d == 4 && gm->status[n - 1] == 1) &&
+            !(d == 5 && gm->status[n - 1] == 2))
+            return fail(MFCFD_EINVAL,
+                        "the last point's split list %d has %d > 22 entries: the reference stores them "
+                        "past the end of its (22,N) array",
+                        d, gm->split_nbhs[d][n - 1]);
+    }
+    // lists that are read: entries must be valid point numbers
+    auto list_used = [&](int d, int i) {
+        const int st = gm->status[i];
+        if (st == 0) return true;
+        if (st == 1) return d <= 3;
+        if (st == 2) return d <= 4;
+        return false;
+    };
+    for (int d = 0; d < 6; ++d)
+        for (int i = 0; i < n; ++i) {
+            if (!list_used(d, i)) continue;
+            for (int j = 0; j < gm->split_nbhs[d][i]; ++j) {
+                const long long off = 22LL * i + j;
+                if (off >= 22LL * n) return fail(MFCFD_EINVAL, "split list %d of point %d runs past the array", d, i + 1);
+                const int k = gm->split_conn[d][off];
+                if (k < 1 || k > n)
+                    return fail(MFCFD_EINVAL, "point %d: split list %d entry %d = %d out of range", i + 1, d, j + 1, k);
+            }
+        }
+
+    c->n = n;
+    hipStream_t st = c->stream;
+    // positions with min_dist packed in .w
+    std::vector<double4> pos4(n);
+    for (int i = 0; i < n; ++i) pos4[i] = make_double4(gm->x[i], gm->y[i], gm->z[i], gm->min_dist[i]);
+    std::vector<double> fr(9ULL * n);
+    for (int i = 0; i < n; ++i)
+        for (int cc = 0; cc < 3; ++cc) {
+            fr[9ULL * i + cc] = gm->tan1[3ULL * i + cc];
+            fr[9ULL * i + 3 + cc] = gm->tan2[3ULL * i + cc];
+            fr[9ULL * i + 6 + cc] = gm->nor[3ULL * i + cc];
+        }
+    std::vector<int> conn0(27ULL * n);
+    for (size_t e = 0; e < conn0.size(); ++e) conn0[e] = gm->conn[e] - 1;
+    const long long stride = 22LL * n + kSplitSlack;
+    std::vector<int> cnt(6ULL * n), split0(6ULL * stride, -1);
+    for (int d = 0; d < 6; ++d) {
+        std::memcpy(&cnt[(size_t)d * n], gm->split_nbhs[d], sizeof(int) * (size_t)n);
+        for (long long e = 0; e < 22LL * n; ++e) split0[d * stride + e] = gm->split_conn[d][e] - 1;
+    }
+    // residual order: wall points, then interior points (state_update.f90:29,62)
+    std::vector<int> respos(n, 0x7fffffff);
+    std::vector<int> li, lw, lo, lsi, lso;
+    for (int i = 0; i < n; ++i) {
+        switch (gm->status[i]) {
+            case 0: li.push_back(i); break;
+            case 1: lw.push_back(i); break;
+            case 2: lo.push_back(i); break;
+            case 5: lsi.push_back(i); break;
+            case 6: lso.push_back(i); break;
+        }
+    }
+    for (size_t m = 0; m < lw.size(); ++m) respos[lw[m]] = (int)m;
+    for (size_t m = 0; m < li.size(); ++m) respos[li[m]] = (int)(lw.size() + m);
+
+    if ((rc = upload(c->pos, pos4.data(), pos4.size(), st))) return rc;
+    if ((rc = upload(c->frame, fr.data(), fr.size(), st))) return rc;
+    if ((rc = upload(c->status, gm->status, (size_t)n, st))) return rc;
+    if ((rc = upload(c->nbhs, gm->nbhs, (size_t)n, st))) return rc;
+    if ((rc = upload(c->conn, conn0.data(), conn0.size(), st))) return rc;
+    if ((rc = upload(c->split_cnt, cnt.data(), cnt.size(), st))) return rc;
+    if ((rc = upload(c->split, split0.data(), split0.size(), st))) return rc;
+    if ((rc = upload(c->alias, gm->alias, (size_t)n, st))) return rc;
+    if ((rc = upload(c->respos, respos.data(), respos.size(), st))) return rc;
+    if ((rc = upload(c->idx_interior, li.data(), li.size(), st))) return rc;
+    if ((rc = upload(c->idx_wall, lw.data(), lw.size(), st))) return rc;
+    if ((rc = upload(c->idx_outer, lo.data(), lo.size(), st))) return rc;
+    if ((rc = upload(c->idx_sup_in, lsi.data(), lsi.size(), st))) return rc;
+    if ((rc = upload(c->idx_sup_out, lso.data(), lso.size(), st))) return rc;
+    c->tl.n_interior = (int)li.size();
+    c->tl.n_wall = (int)lw.size();
+    c->tl.n_outer = (int)lo.size();
+    c->tl.n_sup_in = (int)lsi.size();
+    c->tl.n_sup_out = (int)lso.size();
+    c->tl.interior = c->idx_interior.p;
+    c->tl.wall = c->idx_wall.p;
+    c->tl.outer = c->idx_outer.p;
+    c->tl.sup_in = c->idx_sup_in.p;
+    c->tl.sup_out = c->idx_sup_out.p;
+
+    // flow state (zero until uploaded; initial_conditions.f90:23-24 zero delUp/delUn)
+    DevBuf<double>* st5[] = {&c->prim, &c->U, &c->q, &c->flux_res, &c->delUp, &c->delUn};
+    for (auto* b : st5) {
+        if (b->alloc(5ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "state arrays");
+        HIP_TRY(hipMemsetAsync(b->p, 0, 5ULL * n * sizeof(double), st));
+    }
+    if (c->qm.alloc(10ULL * n) != hipSuccess || c->dq.alloc(15ULL * n) != hipSuccess ||
+        c->delt.alloc((size_t)n) != hipSuccess || c->dinv.alloc((size_t)n) != hipSuccess)
+        return fail(MFCFD_ENOMEM, "state arrays");
+    HIP_TRY(hipMemsetAsync(c->qm.p, 0, 10ULL * n * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(c->dq.p, 0, 15ULL * n * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(c->delt.p, 0, (size_t)n * sizeof(double), st));
+    if (c->fp.inner_iterations > 0 && c->temp.alloc(15ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "temp");
+    c->n_partial = state_update_blocks(n);
+    if (c->partial.alloc(2ULL * c->n_partial) != hipSuccess || c->partial_pos.alloc(2ULL * c->n_partial) != hipSuccess)
+        return fail(MFCFD_ENOMEM, "residual partials");
+
+    // sweep schedule over the alias order
+    std::vector<unsigned char> swept(n, 0);
+    for (int i = 0; i < n; ++i) {
+        const int p = gm->point_with_alias[i];
+        if (p < 0 || p > n) return fail(MFCFD_EINVAL, "point_with_alias(%d)=%d out of range", i + 1, p);
+        if (p > 0 && (gm->status[p - 1] == 0 || gm->status[p - 1] == 1)) swept[p - 1] = 1;
+    }
+    c->implicit_supported = lsi.empty() && lso.empty();
+    Schedule::Inputs in;
+    in.n = n;
+    in.status = gm->status;
+    in.nbhs = gm->nbhs;
+    in.conn = gm->conn;
+    for (int d = 0; d < 6; ++d) {
+        in.split_cnt[d] = gm->split_nbhs[d];
+        in.split[d] = gm->split_conn[d];
+    }
+    in.alias = gm->alias;
+    in.pwa = gm->point_with_alias;
+    in.swept = swept.data();
+    c->sched.build(in);
+    if ((rc = upload(c->fwd_points, c->sched.fwd_points.data(), c->sched.fwd_points.size(), st))) return rc;
+    if ((rc = upload(c->bwd_points, c->sched.bwd_points.data(), c->sched.bwd_points.size(), st))) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    c->have_geom = true;
+    c->res_old = c->res_new = c->residue = 0.0;
+    c->last_t = 0;
+    return MFCFD_OK;
+}
+
+int mfcfd_upload(mfcfd_ctx* c, int id, const double* host) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    ArrayDesc a = array_of(c, id);
+    if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
+    HIP_TRY(hipMemcpyAsync(a.buf->p, host, a.per_point * c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MFCFD_OK;
+}
+
+int mfcfd_download(mfcfd_ctx* c, int id, double* host) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    ArrayDesc a = array_of(c, id);
+    if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
+    HIP_TRY(hipMemcpyAsync(host, a.buf->p, a.per_point * c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MFCFD_OK;
+}
+
+int mfcfd_compute_conserved_vector(mfcfd_ctx* c) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    launch_conserved_vector(c->geom(), c->state(), c->stream);
+    return launch_error("compute_conserved_vector");
+}
+
+int mfcfd_eval_q_variables(mfcfd_ctx* c) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    launch_q_variables(c->geom(), c->state(), c->stream);
+    return launch_error("eval_q_variables");
+}
+
+int mfcfd_eval_q_derivatives(mfcfd_ctx* c) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    launch_q_derivatives(c->geom(), c->state(), c->fp, c->temp.p, c->stream);
+    return launch_error("eval_q_derivatives");
+}
+
+int mfcfd_timestep_delt(mfcfd_ctx* c) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    launch_timestep(c->geom(), c->state(), c->fp, c->stream);
+    return launch_error("timestep_delt");
+}
+
+int mfcfd_eval_flux_residual(mfcfd_ctx* c) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream);
+    return launch_error("eval_flux_residual");
+}
+
+int mfcfd_implicit_forward_backward_sweeps(mfcfd_ctx* c) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    return do_sweeps(c);
+}
+
+int mfcfd_state_update(mfcfd_ctx* c) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if ((rc = do_state_update(c, nullptr, 0))) return rc;
+    return fetch_record(c);
+}
+
+int mfcfd_fpi_solver(mfcfd_ctx* c, int t, double* res_new, double* residue) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if ((rc = do_iteration(c, nullptr, 0))) return rc;
+    if ((rc = fetch_record(c))) return rc;
+    residue_bookkeeping(c, t, c->res_new);
+    if (res_new) *res_new = c->res_new;
+    if (residue) *residue = c->residue;
+    return MFCFD_OK;
+}
+
+int mfcfd_run(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, double* residue_hist) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (n_iters < 0) return fail(MFCFD_EINVAL, "n_iters < 0");
+    if (n_iters == 0) return MFCFD_OK;
+    if (c->hist.n < (size_t)n_iters && c->hist.alloc((size_t)n_iters) != hipSuccess)
+        return fail(MFCFD_ENOMEM, "history buffer");
+    for (int it = 0; it < n_iters; ++it)
+        if ((rc = do_iteration(c, c->hist.p, it))) return rc;
+    std::vector<double> h(n_iters);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->hist.p, sizeof(double) * n_iters, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = fetch_record(c))) return rc;
+    for (int it = 0; it < n_iters; ++it) {
+        residue_bookkeeping(c, t0 + it, h[it]);
+        if (res_new_hist) res_new_hist[it] = c->res_new;
+        if (residue_hist) residue_hist[it] = c->residue;
+    }
+    return MFCFD_OK;
+}
+
+int mfcfd_get_residual(mfcfd_ctx* c, mfcfd_residual* out) {
+    if (!c || !out) return fail(MFCFD_EINVAL, "null argument");
+    out->res_old = c->res_old;
+    out->res_new = c->res_new;
+    out->residue = c->residue;
+    out->max_res = c->max_res;
+    out->sum_res_sqr = c->sum_res_sqr;
+    out->max_res_point = c->max_res_point;
+    out->iteration = c->last_t;
+    return MFCFD_OK;
+}
+
+int mfcfd_synchronize(mfcfd_ctx* c) {
+    if (!c) return fail(MFCFD_EINVAL, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MFCFD_OK;
+}
+
+int mfcfd_sweep_info(mfcfd_ctx* c, int* levels_fwd, int* levels_bwd, int* n_swept) {
+    if (!c || !c->have_geom) return fail(MFCFD_ESTATE, "no geometry bound");
+    if (levels_fwd) *levels_fwd = c->sched.n_levels_fwd;
+    if (levels_bwd) *levels_bwd = c->sched.n_levels_bwd;
+    if (n_swept) *n_swept = (int)c->sched.order.size();
+    return MFCFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// kernels.h -- launch wrappers of the HIP kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_state.h"
+
+namespace mfcfd {
+
+struct TypeLists {
+    int n_interior, n_wall, n_outer, n_sup_in, n_sup_out;
+    const int* interior;  // 0-based point numbers
+    const int* wall;
+    const int* outer;
+    const int* sup_in;
+    const int* sup_out;
+};
+
+struct SweepSchedule {
+    int n_levels_fwd, n_levels_bwd;
+    const int* fwd_ptr;      // host copy: level l covers fwd_points[fwd_ptr[l] .. fwd_ptr[l+1])
+    const int* bwd_ptr;
+    const int* fwd_points;   // device
+    const int* bwd_points;   // device
+    int n_swept;
+};
+
+// compute_conserved_vector.f90 / q_variables.f90
+void launch_conserved_vector(const DevGeom& g, const DevState& s, hipStream_t st);
+void launch_q_variables(const DevGeom& g, const DevState& s, hipStream_t st);
+void launch_conserved_and_q(const DevGeom& g, const DevState& s, hipStream_t st);
+// q_derivatives.f90 / timestep_delt.f90
+void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams& p, double* temp,
+                          hipStream_t st);
+void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
+void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
+// flux_residual.f90
+void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p,
+                          const TypeLists& tl, hipStream_t st);
+// explicit form of the sweep (delUn = -delt*flux_res, delUp = delUn)
+void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st);
+// implicit_forward_and_reverse_sweeps.f90
+void launch_dinv(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
+                 hipStream_t st);
+void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams& p,
+                          const SweepSchedule& sch, hipStream_t st);
+// state_update.f90 (+ residual reduction into *rec; hist may be null)
+void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p,
+                         const TypeLists& tl, double* partial, int* partial_pos, int n_partial,
+                         ResidualRecord* rec, double* hist, int hist_index, hipStream_t st);
+int state_update_blocks(int n);
+
+}  // namespace mfcfd

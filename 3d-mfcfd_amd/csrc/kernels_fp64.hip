@@ -1,0 +1,820 @@
+// kernels_fp64.hip -- FP64 kernels of the q-LSKUM pseudo-time iteration.
+//
+// One thread per point; points of one class (interior, wall, outer) are
+// launched together so a wavefront never mixes stencil shapes.  Expression
+// order follows the reference routine named at each kernel; this file is
+// compiled with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "device_state.h"
+#include "kernels.h"
+#include "qlskum_math.h"
+
+namespace mfcfd {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ Frame<double> load_frame(const double* frame, int k) {
+    Frame<double> f;
+    const double* p = frame + 9 * (long long)k;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        f.t1[c] = p[c];
+        f.t2[c] = p[3 + c];
+        f.n[c] = p[6 + c];
+    }
+    return f;
+}
+
+inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
+
+// ---------------------------------------------------------------------------
+// compute_conserved_vector.f90:20-32 and q_variables.f90:18-38
+// ---------------------------------------------------------------------------
+template <bool DO_U, bool DO_Q>
+__global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    double p[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) p[r] = s.prim[5 * (long long)k + r];
+    if (DO_U) {
+        const Frame<double> f = load_frame(g.frame, k);
+        double U[5];
+        U[0] = p[0];
+        double temp1 = p[0] * p[1];
+        const double temp2 = p[0] * p[2];
+        const double temp3 = p[0] * p[3];
+        U[1] = temp1 * f.t1[0] + temp2 * f.t1[1] + temp3 * f.t1[2];
+        U[2] = temp1 * f.t2[0] + temp2 * f.t2[1] + temp3 * f.t2[2];
+        U[3] = temp1 * f.n[0] + temp2 * f.n[1] + temp3 * f.n[2];
+        temp1 = p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
+        U[4] = 2.5 * p[4] + 0.5 * U[0] * temp1;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) s.U[5 * (long long)k + r] = U[r];
+    }
+    if (DO_Q) {
+        const double rho = p[0], u1 = p[1], u2 = p[2], u3 = p[3], pr = p[4];
+        const double beta = 0.5 * rho / pr;
+        double q[5];
+        q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2 + u3 * u3);
+        const double two_times_beta = 2.0 * beta;
+        q[1] = two_times_beta * u1;
+        q[2] = two_times_beta * u2;
+        q[3] = two_times_beta * u3;
+        q[4] = -two_times_beta;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) s.q[5 * (long long)k + r] = q[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// q_derivatives.f90:17-110 (first pass) and timestep_delt.f90:13-38
+// ---------------------------------------------------------------------------
+template <bool DO_DQ, bool DO_DT>
+__global__ void __launch_bounds__(kBlock) k_qderiv_timestep(DevGeom g, DevState s, FlowParams fp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const double4 pi4 = g.pos[i];
+    const int nb = g.nbhs[i];
+    const int* cn = g.conn + kConnCols * (long long)i;
+    double qi[5], qmax[5], qmin[5];
+    double sx[5], sy[5], sz[5];
+    Geo6<double> geo = {0, 0, 0, 0, 0, 0};
+    if (DO_DQ) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            qi[r] = s.q[5 * (long long)i + r];
+            qmax[r] = qi[r];
+            qmin[r] = qi[r];
+            sx[r] = sy[r] = sz[r] = 0.0;
+        }
+    }
+    double min_delt = 1.0;
+    for (int j = 0; j < nb; ++j) {
+        const int k = cn[j];
+        const double4 pk = g.pos[k];
+        const double delx = pk.x - pi4.x;
+        const double dely = pk.y - pi4.y;
+        const double delz = pk.z - pi4.z;
+        const double dist = __builtin_sqrt(delx * delx + dely * dely + delz * delz);
+        if (DO_DQ) {
+            double qk[5];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) qk[r] = s.q[5 * (long long)k + r];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                if (qk[r] > qmax[r]) qmax[r] = qk[r];
+                if (qk[r] < qmin[r]) qmin[r] = qk[r];
+            }
+            const double weights = 1.0 / pow(dist, fp.power);
+            geo.xx = geo.xx + delx * delx * weights;
+            geo.yy = geo.yy + dely * dely * weights;
+            geo.zz = geo.zz + delz * delz * weights;
+            geo.xy = geo.xy + delx * dely * weights;
+            geo.yz = geo.yz + dely * delz * weights;
+            geo.zx = geo.zx + delz * delx * weights;
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                const double t = qk[r] - qi[r];
+                sx[r] = sx[r] + weights * delx * t;
+                sy[r] = sy[r] + weights * dely * t;
+                sz[r] = sz[r] + weights * delz * t;
+            }
+        }
+        if (DO_DT) {
+            const double* pk5 = s.prim + 5 * (long long)k;
+            const double rho = pk5[0], u1 = pk5[1], u2 = pk5[2], u3 = pk5[3], pr = pk5[4];
+            const double mod_u = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+            const double delt = fp.cfl * dist / (mod_u + 3.0 * __builtin_sqrt(pr / rho));
+            if (delt < min_delt) min_delt = delt;
+        }
+    }
+    if (DO_DQ) {
+        const double det = cramer_det(geo);
+        const double one_by_det = 1.0 / det;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            s.qm[(5 * (long long)i + r) * 2 + 0] = qmax[r];
+            s.qm[(5 * (long long)i + r) * 2 + 1] = qmin[r];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                s.dq[(5 * (long long)i + r) * 3 + c] = cramer_num(geo, c, sx[r], sy[r], sz[r]) * one_by_det;
+        }
+    }
+    if (DO_DT) s.delt[i] = min_delt;
+}
+
+// q_derivatives.f90:116-201: inner iterations (second-order q-derivatives)
+__global__ void __launch_bounds__(kBlock) k_qderiv_inner(DevGeom g, DevState s, FlowParams fp,
+                                                         double* temp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const double4 pi4 = g.pos[i];
+    const int nb = g.nbhs[i];
+    const int* cn = g.conn + kConnCols * (long long)i;
+    double sx[5] = {0, 0, 0, 0, 0}, sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0};
+    Geo6<double> geo = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < nb; ++j) {
+        const int k = cn[j];
+        const double4 pk = g.pos[k];
+        const double delx = pk.x - pi4.x;
+        const double dely = pk.y - pi4.y;
+        const double delz = pk.z - pi4.z;
+        const double dist = __builtin_sqrt(delx * delx + dely * dely + delz * delz);
+        const double weights = 1.0 / pow(dist, fp.power);
+        geo.xx = geo.xx + delx * delx * weights;
+        geo.yy = geo.yy + dely * dely * weights;
+        geo.zz = geo.zz + delz * delz * weights;
+        geo.xy = geo.xy + delx * dely * weights;
+        geo.yz = geo.yz + dely * delz * weights;
+        geo.zx = geo.zx + delz * delx * weights;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const double* dqi = s.dq + (5 * (long long)i + r) * 3;
+            const double* dqk = s.dq + (5 * (long long)k + r) * 3;
+            const double qti = s.q[5 * (long long)i + r] - 0.5 * (delx * dqi[0] + dely * dqi[1] + delz * dqi[2]);
+            const double qtn = s.q[5 * (long long)k + r] - 0.5 * (delx * dqk[0] + dely * dqk[1] + delz * dqk[2]);
+            const double t = qtn - qti;
+            sx[r] = sx[r] + weights * delx * t;
+            sy[r] = sy[r] + weights * dely * t;
+            sz[r] = sz[r] + weights * delz * t;
+        }
+    }
+    const double det = cramer_det(geo);
+    const double one_by_det = 1.0 / det;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            temp[(5 * (long long)i + r) * 3 + c] = cramer_num(geo, c, sx[r], sy[r], sz[r]) * one_by_det;
+}
+
+__global__ void k_copy(double* dst, const double* src, long long n) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// flux stencils: interior_flux_dG*.f90, wall_flux_dG*.f90, outer_flux_dG*.f90
+// ---------------------------------------------------------------------------
+struct Stencil {
+    const int* list;
+    int stride;   // 22 for split lists, 27 for conn
+    int count;
+    int family;   // 0 half-range, 1 wall octant, 2 outer octant
+    int axis;
+    int sign;
+    bool by_one_by_det;
+};
+
+// limited reconstruction at one end of edge (i,k) and its flux in the frame
+// of i (interior_flux_dGxpos.f90:92-104)
+__device__ __forceinline__ void edge_end_flux(const DevGeom& g, const DevState& s, const FlowParams& fp,
+                                              int at, double delx, double dely, double delz,
+                                              const Stencil& st, const Frame<double>& f, double G[5]) {
+    double temp[5], qt[5], prim[5];
+    const double* q = s.q + 5 * (long long)at;
+    const double* dq = s.dq + 15 * (long long)at;
+    const double* qm = s.qm + 10 * (long long)at;
+    double epsi = fp.vl_const * g.pos[at].w;
+    epsi = epsi * epsi * epsi;
+    const double epsi2 = epsi * epsi;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        temp[r] = delx * dq[3 * r] + dely * dq[3 * r + 1] + delz * dq[3 * r + 2];
+        qt[r] = q[r] - 0.5 * temp[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const double phi = venkat_phi(qt[r], q[r], qm[2 * r], qm[2 * r + 1], epsi2);
+        qt[r] = q[r] - 0.5 * phi * temp[r];
+    }
+    qtilde_to_primitive(qt, prim);
+    flux_eval(st.family, st.axis, st.sign, f, prim, fp.pi, G);
+}
+
+__device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
+                           const Stencil& st, const Frame<double>& f, double G[5]) {
+    Geo6<double> geo = {0, 0, 0, 0, 0, 0};
+    double sx[5] = {0, 0, 0, 0, 0}, sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0};
+    const double4 pi4 = g.pos[i];
+    const int* list = st.list + (long long)st.stride * i;
+    for (int j = 0; j < st.count; ++j) {
+        const int k = list[j];
+        const double4 pk = g.pos[k];
+        const double delx = pk.x - pi4.x;
+        const double dely = pk.y - pi4.y;
+        const double delz = pk.z - pi4.z;
+        const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+        const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+        const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+        const double dist = __builtin_sqrt(dels * dels + delt * delt + deln * deln);
+        const double weights = 1.0 / pow(dist, fp.power);
+        const double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
+        geo.xx = geo.xx + dels * dels_w;
+        geo.yy = geo.yy + delt * delt_w;
+        geo.zz = geo.zz + deln * deln_w;
+        geo.xy = geo.xy + dels * delt_w;
+        geo.yz = geo.yz + delt * deln_w;
+        geo.zx = geo.zx + deln * dels_w;
+        double Gi[5], Gk[5];
+        edge_end_flux(g, s, fp, i, delx, dely, delz, st, f, Gi);
+        edge_end_flux(g, s, fp, k, delx, dely, delz, st, f, Gk);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const double t = Gk[r] - Gi[r];
+            sx[r] = sx[r] + t * dels_w;
+            sy[r] = sy[r] + t * delt_w;
+            sz[r] = sz[r] + t * deln_w;
+        }
+    }
+    const double det = cramer_det(geo);
+    if (st.by_one_by_det) {
+        const double one_by_det = 1.0 / det;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) G[r] = cramer_num(geo, st.axis, sx[r], sy[r], sz[r]) * one_by_det;
+    } else {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) G[r] = cramer_num(geo, st.axis, sx[r], sy[r], sz[r]) / det;
+    }
+}
+
+__device__ __forceinline__ Stencil split_stencil(const DevGeom& g, int dir, int i, int family) {
+    Stencil st;
+    st.list = g.split + g.split_stride * dir;
+    st.stride = kSplitCols;
+    st.count = g.split_cnt[(long long)dir * g.n + i];
+    st.family = family;
+    st.axis = dir >> 1;
+    st.sign = (dir & 1) ? -1 : 1;
+    st.by_one_by_det = family == 2;
+    return st;
+}
+
+__device__ __forceinline__ Stencil conn_stencil(const DevGeom& g, int i, int axis, int sign) {
+    Stencil st;
+    st.list = g.conn;
+    st.stride = kConnCols;
+    st.count = g.nbhs[i];
+    st.family = 0;
+    st.axis = axis;
+    st.sign = sign;
+    st.by_one_by_det = false;
+    return st;
+}
+
+// flux_residual.f90:40-81; CLASS 0 interior, 1 wall, 2 outer
+template <int CLASS>
+__global__ void __launch_bounds__(kBlock) k_flux_residual(DevGeom g, DevState s, FlowParams fp,
+                                                          const int* idx, int count) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= count) return;
+    const int i = idx[m];
+    const Frame<double> f = load_frame(g.frame, i);
+    double acc[5], G[5];
+    if (CLASS == 0) {
+        stencil_dG(g, s, fp, i, split_stencil(g, XPOS, i, 0), f, acc);
+        for (int d = XNEG; d <= ZNEG; ++d) {
+            stencil_dG(g, s, fp, i, split_stencil(g, d, i, 0), f, G);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
+        }
+    } else if (CLASS == 1) {
+        stencil_dG(g, s, fp, i, split_stencil(g, XPOS, i, 1), f, acc);
+        for (int d = XNEG; d <= YNEG; ++d) {
+            stencil_dG(g, s, fp, i, split_stencil(g, d, i, 1), f, G);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
+        }
+        // wall_flux_dGzneg.f90:61-65: full conn, half-range flux_Gzn
+        stencil_dG(g, s, fp, i, conn_stencil(g, i, 2, -1), f, G);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) acc[r] = 2.0 * (acc[r] + G[r]);
+    } else {
+        stencil_dG(g, s, fp, i, split_stencil(g, XPOS, i, 2), f, acc);
+        for (int d = XNEG; d <= YNEG; ++d) {
+            stencil_dG(g, s, fp, i, split_stencil(g, d, i, 2), f, G);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
+        }
+        // outer_flux_dGzpos.f90: zpos list, flux_Gzp, temp*(1/det)
+        Stencil st = split_stencil(g, ZPOS, i, 0);
+        st.by_one_by_det = true;
+        stencil_dG(g, s, fp, i, st, f, G);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) s.flux_res[5 * (long long)i + r] = acc[r];
+}
+
+// explicit update through the sweep's outputs: delUn = -delt*R, delUp = delUn
+__global__ void __launch_bounds__(kBlock) k_explicit_sweep(DevGeom g, DevState s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    const double dt = s.delt[k];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const double v = -(dt * s.flux_res[5 * (long long)k + r]);
+        s.delUn[5 * (long long)k + r] = v;
+        s.delUp[5 * (long long)k + r] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// implicit LU-SGS: implicit_D_inv_{interior,wall}.f90 and the alias-ordered
+// sweeps of implicit_forward_and_reverse_sweeps.f90:26-88
+// ---------------------------------------------------------------------------
+__device__ double dinv_dir(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
+                           const Stencil& st, const Frame<double>& f, double as) {
+    Geo6<double> geo = {0, 0, 0, 0, 0, 0};
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const double4 pi4 = g.pos[i];
+    const int* list = st.list + (long long)st.stride * i;
+    for (int j = 0; j < st.count; ++j) {
+        const int k = list[j];
+        const double4 pk = g.pos[k];
+        const double delx = pk.x - pi4.x;
+        const double dely = pk.y - pi4.y;
+        const double delz = pk.z - pi4.z;
+        const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+        const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+        const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+        const double dist = __builtin_sqrt(dels * dels + delt * delt + deln * deln);
+        const double weights = 1.0 / pow(dist, fp.power);
+        const double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
+        geo.xx = geo.xx + dels * dels_w;
+        geo.yy = geo.yy + delt * delt_w;
+        geo.zz = geo.zz + deln * deln_w;
+        geo.xy = geo.xy + dels * delt_w;
+        geo.yz = geo.yz + delt * deln_w;
+        geo.zx = geo.zx + deln * dels_w;
+        sx = sx + dels_w;
+        sy = sy + delt_w;
+        sz = sz + deln_w;
+    }
+    const double det = cramer_det(geo);
+    const double temp = cramer_num(geo, st.axis, sx, sy, sz);
+    // global-frame velocity component with the sound speed
+    // (implicit_D_inv_interior.f90:107-108,168-169)
+    const double vel = s.prim[5 * (long long)i + 1 + st.axis];
+    if (st.sign > 0) {
+        const double temp1 = vel + as;
+        return 0.50 * fabs(temp1 + fabs(temp1)) * temp / det;
+    }
+    const double temp1 = vel - as;
+    return -(0.50 * fabs(temp1 - fabs(temp1)) * temp / det);
+}
+
+__global__ void __launch_bounds__(kBlock) k_dinv(DevGeom g, DevState s, FlowParams fp, const int* idx,
+                                                 int count) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= count) return;
+    const int i = idx[m];
+    const int st_i = g.status[i];
+    const Frame<double> f = load_frame(g.frame, i);
+    const double delta_t = s.delt[i];
+    const double as = __builtin_sqrt(1.40 * s.prim[5 * (long long)i + 4] / s.prim[5 * (long long)i]);
+    double d[6];
+    double temp;
+    if (st_i == 1) {
+        for (int dd = XPOS; dd <= YNEG; ++dd) d[dd] = dinv_dir(g, s, fp, i, split_stencil(g, dd, i, 1), f, as);
+        d[4] = dinv_dir(g, s, fp, i, conn_stencil(g, i, 2, -1), f, as);
+        temp = (1.0 / delta_t) - 2.0 * (d[0] + d[1] + d[2] + d[3] + d[4]);
+    } else {
+        for (int dd = XPOS; dd <= ZNEG; ++dd) d[dd] = dinv_dir(g, s, fp, i, split_stencil(g, dd, i, 0), f, as);
+        temp = (1.0 / delta_t) - (d[0] + d[1] + d[2] + d[3] + d[4] + d[5]);
+    }
+    s.dinv[i] = 1.0 / temp;
+}
+
+// one direction of LdelU (FORWARD) or UdelU
+// (implicit_LdelU_forward_sweep_interior.f90:51-117)
+template <bool FORWARD>
+__device__ void sweep_dir(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
+                          const Stencil& st, const Frame<double>& f, double dG[5]) {
+    Geo6<double> geo = {0, 0, 0, 0, 0, 0};
+    double sx[5] = {0, 0, 0, 0, 0}, sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0};
+    const double4 pi4 = g.pos[i];
+    const int alias_i = g.alias[i];
+    const int* list = st.list + (long long)st.stride * i;
+    const double* dU = FORWARD ? s.delUp : s.delUn;
+    for (int j = 0; j < st.count; ++j) {
+        const int k = list[j];
+        const double4 pk = g.pos[k];
+        const double delx = pk.x - pi4.x;
+        const double dely = pk.y - pi4.y;
+        const double delz = pk.z - pi4.z;
+        const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+        const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+        const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+        const double dist = __builtin_sqrt(dels * dels + delt * delt + deln * deln);
+        const double weights = 1.0 / pow(dist, fp.power);
+        const double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
+        geo.xx = geo.xx + dels * dels_w;
+        geo.yy = geo.yy + delt * delt_w;
+        geo.zz = geo.zz + deln * deln_w;
+        geo.xy = geo.xy + dels * delt_w;
+        geo.yz = geo.yz + delt * deln_w;
+        geo.zx = geo.zx + deln * dels_w;
+        const int alias_k = g.alias[k];
+        const bool take = FORWARD ? (alias_k < alias_i) : (alias_k > alias_i);
+        if (take) {
+            double prim[5], Gold[5], Gnew[5], U[5];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) prim[r] = s.prim[5 * (long long)k + r];
+            flux_eval(st.family, st.axis, st.sign, f, prim, fp.pi, Gold);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) U[r] = dU[5 * (long long)k + r] + s.U[5 * (long long)k + r];
+            const Frame<double> fk = load_frame(g.frame, k);
+            conserved_to_prim(U, fk, prim);
+            flux_eval(st.family, st.axis, st.sign, f, prim, fp.pi, Gnew);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                const double t = Gnew[r] - Gold[r];
+                sx[r] = sx[r] + t * dels_w;
+                sy[r] = sy[r] + t * delt_w;
+                sz[r] = sz[r] + t * deln_w;
+            }
+        }
+    }
+    const double det = cramer_det(geo);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) dG[r] = cramer_num(geo, st.axis, sx[r], sy[r], sz[r]) / det;
+}
+
+template <bool FORWARD>
+__device__ void sweep_term(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
+                           const Frame<double>& f, double out[5]) {
+    double d[5];
+    if (g.status[i] == 1) {
+        sweep_dir<FORWARD>(g, s, fp, i, split_stencil(g, XPOS, i, 1), f, out);
+        for (int dd = XNEG; dd <= YNEG; ++dd) {
+            sweep_dir<FORWARD>(g, s, fp, i, split_stencil(g, dd, i, 1), f, d);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) out[r] = out[r] + d[r];
+        }
+        sweep_dir<FORWARD>(g, s, fp, i, conn_stencil(g, i, 2, -1), f, d);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) out[r] = 2.0 * (out[r] + d[r]);
+        return;
+    }
+    sweep_dir<FORWARD>(g, s, fp, i, split_stencil(g, XPOS, i, 0), f, out);
+    for (int dd = XNEG; dd <= ZNEG; ++dd) {
+        sweep_dir<FORWARD>(g, s, fp, i, split_stencil(g, dd, i, 0), f, d);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) out[r] = out[r] + d[r];
+    }
+}
+
+template <bool FORWARD>
+__global__ void __launch_bounds__(64) k_sweep_level(DevGeom g, DevState s, FlowParams fp,
+                                                    const int* points, int begin, int end) {
+    const int m = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= end) return;
+    const int p = points[m];
+    const Frame<double> f = load_frame(g.frame, p);
+    double T5[5];
+    sweep_term<FORWARD>(g, s, fp, p, f, T5);
+    const double D_inv = s.dinv[p];
+    if (FORWARD) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+            s.delUp[5 * (long long)p + r] = -(D_inv * (s.flux_res[5 * (long long)p + r] + T5[r]));
+    } else {
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+            s.delUn[5 * (long long)p + r] = s.delUp[5 * (long long)p + r] - D_inv * T5[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// state_update.f90:22-115 with the residual of state_update.f90:39-44,115
+// ---------------------------------------------------------------------------
+__device__ void conserved_vector_Ubar(const DevGeom& g, const DevState& s, const FlowParams& fp, int k,
+                                      const Frame<double>& f, double Ubar[5]) {
+    // conserved_vector_Ubar.f90:19-58
+    const double pi = fp.pi;
+    const double u1_inf_rot = fp.u1_inf * f.t1[0] + fp.u2_inf * f.t1[1] + fp.u3_inf * f.t1[2];
+    const double u2_inf_rot = fp.u1_inf * f.t2[0] + fp.u2_inf * f.t2[1] + fp.u3_inf * f.t2[2];
+    const double u3_inf_rot = fp.u1_inf * f.n[0] + fp.u2_inf * f.n[1] + fp.u3_inf * f.n[2];
+    const double rho_e_inf = fp.pr_inf * 2.5 + 0.5 * fp.rho_inf *
+        (u1_inf_rot * u1_inf_rot + u2_inf_rot * u2_inf_rot + u3_inf_rot * u3_inf_rot);
+    double beta = (0.5 * fp.rho_inf) / fp.pr_inf;
+    double S3 = u3_inf_rot * __builtin_sqrt(beta);
+    const double B3_inf = exp(-S3 * S3) / (2.0 * __builtin_sqrt(pi * beta));
+    const double A3n_inf = 0.5 * (1.0 - erf(S3));
+    const double* p = s.prim + 5 * (long long)k;
+    const double rho = p[0], u1 = p[1], u2 = p[2], u3 = p[3], pr = p[4];
+    const double u1_rot = u1 * f.t1[0] + u2 * f.t1[1] + u3 * f.t1[2];
+    const double u2_rot = u1 * f.t2[0] + u2 * f.t2[1] + u3 * f.t2[2];
+    const double u3_rot = u1 * f.n[0] + u2 * f.n[1] + u3 * f.n[2];
+    const double rho_e = pr * 2.5 + 0.5 * rho * (u1_rot * u1_rot + u2_rot * u2_rot + u3_rot * u3_rot);
+    beta = (0.5 * rho) / pr;
+    S3 = u3_rot * __builtin_sqrt(beta);
+    const double B3 = exp(-S3 * S3) / (2.0 * __builtin_sqrt(pi * beta));
+    const double A3p = 0.5 * (1.0 + erf(S3));
+    Ubar[0] = (fp.rho_inf * A3n_inf) + (rho * A3p);
+    Ubar[1] = (fp.rho_inf * u1_inf_rot * A3n_inf) + (rho * u1_rot * A3p);
+    Ubar[2] = (fp.rho_inf * u2_inf_rot * A3n_inf) + (rho * u2_rot * A3p);
+    double temp1 = fp.rho_inf * (u3_inf_rot * A3n_inf - B3_inf);
+    double temp2 = rho * (u3_rot * A3p + B3);
+    Ubar[3] = temp1 + temp2;
+    temp1 = (rho_e_inf * A3n_inf - 0.5 * fp.rho_inf * u3_inf_rot * B3_inf);
+    temp2 = (rho_e * A3p + 0.5 * rho * u3_rot * B3);
+    Ubar[4] = temp1 + temp2;
+}
+
+struct MaxPair {
+    double v;
+    int pos;   // position in the reference's wall-then-interior order
+    int point;
+};
+
+__device__ __forceinline__ MaxPair max_pair(MaxPair a, MaxPair b) {
+    if (b.v > a.v) return b;
+    if (b.v == a.v && b.pos < a.pos) return b;
+    return a;
+}
+
+__global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, FlowParams fp,
+                                                         double* partial, int* partial_pos,
+                                                         int* partial_pt, double* partial_max) {
+    __shared__ double ssum[kBlock];
+    __shared__ double smax[kBlock];
+    __shared__ int spos[kBlock];
+    __shared__ int spt[kBlock];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double res_sqr = 0.0;
+    MaxPair mp = {0.0, 0x7fffffff, 0};
+    if (k < g.n) {
+        const int st = g.status[k];
+        if (st == 0 || st == 1) {
+            const Frame<double> f = load_frame(g.frame, k);
+            const double temp = s.U[5 * (long long)k];
+            double U[5];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) U[r] = s.U[5 * (long long)k + r] + s.delUn[5 * (long long)k + r];
+            if (st == 1) U[3] = 0.0;
+            res_sqr = (U[0] - temp) * (U[0] - temp);
+            mp.v = res_sqr;
+            mp.pos = g.respos[k];
+            mp.point = k + 1;
+            double prim[5];
+            conserved_to_prim(U, f, prim);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                s.prim[5 * (long long)k + r] = prim[r];
+                s.delUp[5 * (long long)k + r] = s.delUn[5 * (long long)k + r];
+            }
+        } else if (st == 2) {
+            const Frame<double> f = load_frame(g.frame, k);
+            double U[5], prim[5];
+            conserved_vector_Ubar(g, s, fp, k, f, U);
+            const double dt = s.delt[k];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) U[r] = U[r] - dt * s.flux_res[5 * (long long)k + r];
+            conserved_to_prim(U, f, prim);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = prim[r];
+        }
+    }
+    ssum[threadIdx.x] = res_sqr;
+    smax[threadIdx.x] = mp.v;
+    spos[threadIdx.x] = mp.pos;
+    spt[threadIdx.x] = mp.point;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + w];
+            MaxPair a = {smax[threadIdx.x], spos[threadIdx.x], spt[threadIdx.x]};
+            MaxPair b = {smax[threadIdx.x + w], spos[threadIdx.x + w], spt[threadIdx.x + w]};
+            a = max_pair(a, b);
+            smax[threadIdx.x] = a.v;
+            spos[threadIdx.x] = a.pos;
+            spt[threadIdx.x] = a.point;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = ssum[0];
+        partial_max[blockIdx.x] = smax[0];
+        partial_pos[blockIdx.x] = spos[0];
+        partial_pt[blockIdx.x] = spt[0];
+    }
+}
+
+// state_update.f90:80-114: supersonic points copy the nearest neighbour's
+// state after every other point is updated (outlet list, then inlet list,
+// each in order, so a single thread walks them)
+__global__ void k_supersonic(DevGeom g, DevState s, const int* out_idx, int n_out, const int* in_idx,
+                             int n_in) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int* idx = pass == 0 ? out_idx : in_idx;
+        const int cnt = pass == 0 ? n_out : n_in;
+        for (int m = 0; m < cnt; ++m) {
+            const int k = idx[m];
+            const double4 pk = g.pos[k];
+            double min_dist = 100000.0;
+            int p = k;
+            for (int r = 0; r < g.nbhs[k]; ++r) {
+                const int nbh = g.conn[kConnCols * (long long)k + r];
+                const double4 pn = g.pos[nbh];
+                const double dx = pn.x - pk.x, dy = pn.y - pk.y, dz = pn.z - pk.z;
+                const double ds = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+                if (ds < min_dist) {
+                    min_dist = ds;
+                    p = nbh;
+                }
+            }
+            for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = s.prim[5 * (long long)p + r];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_residual_final(const double* partial, const int* partial_pos,
+                                                         const int* partial_pt, const double* partial_max,
+                                                         int n_partial, int n_points,
+                                                         ResidualRecord* rec, double* hist, int hist_index) {
+    __shared__ double ssum[1024];
+    __shared__ double smax[1024];
+    __shared__ int spos[1024];
+    __shared__ int spt[1024];
+    double acc = 0.0;
+    MaxPair mp = {0.0, 0x7fffffff, 0};
+    // fixed assignment of partials to threads and fixed tree: deterministic
+    for (int b = threadIdx.x; b < n_partial; b += blockDim.x) {
+        acc = acc + partial[b];
+        MaxPair o = {partial_max[b], partial_pos[b], partial_pt[b]};
+        mp = max_pair(mp, o);
+    }
+    ssum[threadIdx.x] = acc;
+    smax[threadIdx.x] = mp.v;
+    spos[threadIdx.x] = mp.pos;
+    spt[threadIdx.x] = mp.point;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + w];
+            MaxPair a = {smax[threadIdx.x], spos[threadIdx.x], spt[threadIdx.x]};
+            MaxPair b = {smax[threadIdx.x + w], spos[threadIdx.x + w], spt[threadIdx.x + w]};
+            a = max_pair(a, b);
+            smax[threadIdx.x] = a.v;
+            spos[threadIdx.x] = a.pos;
+            spt[threadIdx.x] = a.point;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double sum = ssum[0];
+        rec->sum_res_sqr = sum;
+        rec->res_new = __builtin_sqrt(sum) / n_points;
+        rec->max_res = smax[0];
+        // the reference keeps the previous max_res_point when no residual is
+        // positive; 0 here means "unchanged"
+        rec->max_res_point = smax[0] > 0.0 ? spt[0] : 0;
+        if (hist) hist[hist_index] = rec->res_new;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+void launch_conserved_vector(const DevGeom& g, const DevState& s, hipStream_t st) {
+    if (g.n <= 0) return;
+    hipLaunchKernelGGL((k_conserved_q<true, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
+}
+void launch_q_variables(const DevGeom& g, const DevState& s, hipStream_t st) {
+    if (g.n <= 0) return;
+    hipLaunchKernelGGL((k_conserved_q<false, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
+}
+void launch_conserved_and_q(const DevGeom& g, const DevState& s, hipStream_t st) {
+    if (g.n <= 0) return;
+    hipLaunchKernelGGL((k_conserved_q<true, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
+}
+void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams& p, double* temp,
+                          hipStream_t st) {
+    if (g.n <= 0) return;
+    hipLaunchKernelGGL((k_qderiv_timestep<true, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
+                       g, s, p);
+    for (int it = 0; it < p.inner_iterations; ++it) {
+        hipLaunchKernelGGL(k_qderiv_inner, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s, p, temp);
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(15LL * g.n, kBlock)), dim3(kBlock), 0, st, s.dq,
+                           (const double*)temp, 15LL * g.n);
+    }
+}
+void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
+    if (g.n <= 0) return;
+    hipLaunchKernelGGL((k_qderiv_timestep<false, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
+                       g, s, p);
+}
+void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
+    if (g.n <= 0) return;
+    hipLaunchKernelGGL((k_qderiv_timestep<true, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
+                       g, s, p);
+}
+void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
+                          hipStream_t st) {
+    if (tl.n_wall > 0)
+        hipLaunchKernelGGL(k_flux_residual<1>, dim3(grid_for(tl.n_wall, kBlock)), dim3(kBlock), 0, st, g, s,
+                           p, tl.wall, tl.n_wall);
+    if (tl.n_outer > 0)
+        hipLaunchKernelGGL(k_flux_residual<2>, dim3(grid_for(tl.n_outer, kBlock)), dim3(kBlock), 0, st, g,
+                           s, p, tl.outer, tl.n_outer);
+    if (tl.n_interior > 0)
+        hipLaunchKernelGGL(k_flux_residual<0>, dim3(grid_for(tl.n_interior, kBlock)), dim3(kBlock), 0, st,
+                           g, s, p, tl.interior, tl.n_interior);
+}
+void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st) {
+    if (g.n <= 0) return;
+    hipLaunchKernelGGL(k_explicit_sweep, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
+}
+void launch_dinv(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
+                 hipStream_t st) {
+    if (tl.n_wall > 0)
+        hipLaunchKernelGGL(k_dinv, dim3(grid_for(tl.n_wall, kBlock)), dim3(kBlock), 0, st, g, s, p, tl.wall,
+                           tl.n_wall);
+    if (tl.n_interior > 0)
+        hipLaunchKernelGGL(k_dinv, dim3(grid_for(tl.n_interior, kBlock)), dim3(kBlock), 0, st, g, s, p,
+                           tl.interior, tl.n_interior);
+}
+void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepSchedule& sch,
+                          hipStream_t st) {
+    for (int l = 0; l < sch.n_levels_fwd; ++l) {
+        const int b = sch.fwd_ptr[l], e = sch.fwd_ptr[l + 1];
+        hipLaunchKernelGGL(k_sweep_level<true>, dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
+                           sch.fwd_points, b, e);
+    }
+    for (int l = 0; l < sch.n_levels_bwd; ++l) {
+        const int b = sch.bwd_ptr[l], e = sch.bwd_ptr[l + 1];
+        hipLaunchKernelGGL(k_sweep_level<false>, dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
+                           sch.bwd_points, b, e);
+    }
+    // implicit_forward_and_reverse_sweeps.f90:86-88
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,
+                       (const double*)s.delUn, 5LL * g.n);
+}
+int state_update_blocks(int n) { return grid_for(n, kBlock); }
+void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
+                         double* partial, int* partial_pos, int n_partial, ResidualRecord* rec, double* hist,
+                         int hist_index, hipStream_t st) {
+    // partial buffer layout: [sum | max] doubles, [pos | point] ints
+    double* partial_max = partial + n_partial;
+    int* partial_pt = partial_pos + n_partial;
+    hipLaunchKernelGGL(k_state_update, dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial, partial_pos,
+                       partial_pt, partial_max);
+    if (tl.n_sup_in + tl.n_sup_out > 0)
+        hipLaunchKernelGGL(k_supersonic, dim3(1), dim3(1), 0, st, g, s, tl.sup_out, tl.n_sup_out, tl.sup_in,
+                           tl.n_sup_in);
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, st, (const double*)partial,
+                       (const int*)partial_pos, (const int*)partial_pt, (const double*)partial_max, n_partial,
+                       g.n, rec, hist, hist_index);
+}
+
+}  // namespace mfcfd

@@ -1,0 +1,153 @@
+/*
+ * mfcfd_hip.h -- C ABI of libmfcfd_hip.so, the MI355X implementation of the
+ * q-LSKUM pseudo-time iteration of Pathlessbark8/3d-mfcfd.
+ *
+ * The reference's hot path is a set of argument-free Fortran module
+ * procedures that read and write the module globals of data_structure_mod
+ * (data_structure.f90:46-59) and the PARAMETERs of parameter_mod
+ * (parameter.f90:8-33).  This ABI carries the same arrays in the same
+ * layout -- column-major (m,N) arrays passed as plain pointers, 1-based
+ * point numbers in every index array -- so a Fortran host binds it with
+ * iso_c_binding (see fortran/mfcfd_dropin.f90) and keeps its own
+ * allocatables.  Device copies live in HBM for the life of a context; host
+ * arrays are refreshed only by mfcfd_download().
+ *
+ * Every function returns 0 on success or a negative MFCFD_E* code; the
+ * message is available from mfcfd_last_error().  Nothing throws across the
+ * ABI.  A context is not re-entrant: one host thread drives it, and calls
+ * run in order on the context's HIP stream.
+ */
+#ifndef MFCFD_HIP_H
+#define MFCFD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFCFD_ABI_VERSION 1
+
+/* error codes */
+#define MFCFD_OK 0
+#define MFCFD_EINVAL (-1)      /* bad argument or inconsistent arrays */
+#define MFCFD_ESTATE (-2)      /* call out of order (e.g. no geometry bound) */
+#define MFCFD_EHIP (-3)        /* HIP runtime error */
+#define MFCFD_ENOMEM (-4)      /* device or host allocation failed */
+#define MFCFD_EUNSUPPORTED (-5)
+
+/* iteration forms */
+#define MFCFD_EXPLICIT 0       /* U <- U - delt*R (state_update.f90:35,67) */
+#define MFCFD_IMPLICIT 1       /* LU-SGS sweeps (implicit_forward_and_reverse_sweeps.f90) */
+#define MFCFD_FP64 0
+#define MFCFD_MIXED 1
+
+/* array ids for upload/download; shapes are the reference's */
+#define MFCFD_ARR_PRIM 0       /* prim(5,N) */
+#define MFCFD_ARR_U 1          /* U(5,N) */
+#define MFCFD_ARR_Q 2          /* q(5,N) */
+#define MFCFD_ARR_QM 3         /* qm(2,5,N) */
+#define MFCFD_ARR_DQ 4         /* dq(3,5,N) */
+#define MFCFD_ARR_DELT 5       /* delt(N) */
+#define MFCFD_ARR_FLUX_RES 6   /* flux_res(5,N) */
+#define MFCFD_ARR_DELUP 7      /* delUp(5,N) */
+#define MFCFD_ARR_DELUN 8      /* delUn(5,N) */
+#define MFCFD_ARR_COUNT 9
+
+typedef struct mfcfd_ctx mfcfd_ctx;
+
+/* parameter.f90:8-33, runtime instead of compile time */
+typedef struct mfcfd_params {
+    double mach;             /* Mach */
+    double aoa;              /* aoa, degrees */
+    double cfl;              /* CFL */
+    double vl_const;         /* VL_CONST (Venkatakrishnan constant) */
+    double power;            /* power (least-squares weight exponent) */
+    int inner_iterations;    /* inner_iterations (second-order q-derivatives) */
+    int mode;                /* MFCFD_EXPLICIT or MFCFD_IMPLICIT */
+    int precision;           /* MFCFD_FP64 or MFCFD_MIXED */
+    int max_iters;           /* max_iters (used by the Fortran drop-in's final download) */
+} mfcfd_params;
+
+/* the point/connectivity arrays of data_structure.f90:11-28,49-56 as the
+ * host holds them (all point numbers 1-based) */
+typedef struct mfcfd_geometry {
+    int n;                          /* max_points */
+    const double *x, *y, *z;        /* (N) */
+    const double *tan1, *tan2, *nor;/* (3,N) */
+    const double *min_dist;         /* (N) */
+    const int *status;              /* (N): 0 interior, 1 wall, 2 outer, 5/6 supersonic */
+    const int *nbhs;                /* (N), each <= 27 */
+    const int *conn;                /* (27,N) */
+    const int *split_nbhs[6];       /* xpos,xneg,ypos,yneg,zpos,zneg _nbhs (N) */
+    const int *split_conn[6];       /* xpos..zneg _conn (22,N), exactly as the host
+                                       built them: lists longer than 22 continue
+                                       into the next point's column */
+    const int *alias;               /* (N) implicit_aliasing.f90 */
+    const int *point_with_alias;    /* (N) */
+} mfcfd_geometry;
+
+typedef struct mfcfd_residual {
+    double res_old, res_new, residue;   /* data_structure.f90:58 */
+    double max_res;                     /* largest (U1 change)^2 */
+    double sum_res_sqr;
+    int max_res_point;                  /* 1-based */
+    int iteration;                      /* t of the last fpi_solver call */
+} mfcfd_residual;
+
+/* ---- context ---------------------------------------------------------- */
+int mfcfd_create(int device, mfcfd_ctx **out);
+int mfcfd_destroy(mfcfd_ctx *ctx);
+const char *mfcfd_last_error(void);
+int mfcfd_abi_version(void);
+
+/* parameter_mod (parameter.f90:8-33) */
+int mfcfd_set_params(mfcfd_ctx *ctx, const mfcfd_params *p);
+
+/* copies the point arrays to HBM once; replaces the host->module-global
+ * handoff after read_input_point_data (point_preprocessor.f90:13-98),
+ * generate_split_stencils (generate_connectivity.f90:12-34) and aliasing
+ * (implicit_aliasing.f90:13-44).  Also builds the sweep schedule. */
+int mfcfd_bind_geometry(mfcfd_ctx *ctx, const mfcfd_geometry *g);
+
+/* host preprocessing drop-ins: generate_split_stencils
+ * (generate_connectivity.f90:12-246) and aliasing (implicit_aliasing.f90:
+ * 13-44) on the caller's arrays.  split_conn buffers must hold 22*N+32 ints
+ * (lists longer than 22 spill as in the reference). */
+int mfcfd_generate_split_stencils(int n, const double *x, const double *y, const double *z,
+                                  const double *tan1, const double *tan2, const double *nor,
+                                  const int *status, const int *nbhs, const int *conn,
+                                  int *split_nbhs[6], int *split_conn[6]);
+int mfcfd_aliasing(int n, const int *nbhs, const int *conn, int *alias, int *point_with_alias);
+
+/* state transfer by array id (MFCFD_ARR_*), reference layout */
+int mfcfd_upload(mfcfd_ctx *ctx, int array_id, const double *host);
+int mfcfd_download(mfcfd_ctx *ctx, int array_id, double *host);
+
+/* ---- the hot path: one call per reference entry point ----------------- */
+int mfcfd_compute_conserved_vector(mfcfd_ctx *ctx); /* compute_conserved_vector.f90:13 */
+int mfcfd_eval_q_variables(mfcfd_ctx *ctx);         /* q_variables.f90:10 */
+int mfcfd_eval_q_derivatives(mfcfd_ctx *ctx);       /* q_derivatives.f90:10 */
+int mfcfd_timestep_delt(mfcfd_ctx *ctx);            /* timestep_delt.f90:8 */
+int mfcfd_eval_flux_residual(mfcfd_ctx *ctx);       /* flux_residual.f90:30 */
+int mfcfd_implicit_forward_backward_sweeps(mfcfd_ctx *ctx); /* implicit_forward_and_reverse_sweeps.f90:16;
+                                                       explicit mode: delUn = -delt*flux_res */
+int mfcfd_state_update(mfcfd_ctx *ctx);             /* state_update.f90:15 */
+
+/* fpi_solver.f90:20-45: the whole iteration plus the residue bookkeeping;
+ * res_new/residue may be NULL */
+int mfcfd_fpi_solver(mfcfd_ctx *ctx, int t, double *res_new, double *residue);
+
+/* q_lskum.f90:28-32 without the printing: iterations t0..t0+n-1 with no host
+ * round trip; per-iteration res_new and residue (either may be NULL) */
+int mfcfd_run(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, double *residue_hist);
+
+int mfcfd_get_residual(mfcfd_ctx *ctx, mfcfd_residual *out);
+int mfcfd_synchronize(mfcfd_ctx *ctx);
+
+/* diagnostics: sweep schedule depth (levels of the forward and backward
+ * Gauss-Seidel dependency chains) and swept point count */
+int mfcfd_sweep_info(mfcfd_ctx *ctx, int *levels_fwd, int *levels_bwd, int *n_swept);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -629,6 +629,53 @@ int mfcfd_synchronize(mfcfd_ctx* c) {
     return MFCFD_OK;
 }
 
+int mfcfd_time_phases(mfcfd_ctx* c, int t0, int n_iters, double* out_ms) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (n_iters <= 0 || !out_ms) return fail(MFCFD_EINVAL, "n_iters must be positive and out_ms non-null");
+    hipEvent_t ev[MFCFD_PHASE_COUNT + 1];
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    double acc[MFCFD_PHASE_COUNT] = {0, 0, 0, 0, 0, 0};
+    const DevGeom g = c->geom();
+    const DevState s = c->state();
+    for (int it = 0; it < n_iters && rc == 0; ++it) {
+        hipEventRecord(ev[0], c->stream);
+        launch_conserved_and_q(g, s, c->stream);
+        hipEventRecord(ev[1], c->stream);
+        if (c->fp.inner_iterations > 0) {
+            launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
+            launch_timestep(g, s, c->fp, c->stream);
+        } else {
+            launch_qderiv_timestep(g, s, c->fp, c->stream);
+        }
+        hipEventRecord(ev[2], c->stream);
+        launch_flux_residual(g, s, c->fp, c->tl, c->stream, 1);
+        hipEventRecord(ev[3], c->stream);
+        launch_flux_residual(g, s, c->fp, c->tl, c->stream, 6);
+        hipEventRecord(ev[4], c->stream);
+        if ((rc = launch_error("phases"))) break;
+        if ((rc = do_sweeps(c))) break;
+        hipEventRecord(ev[5], c->stream);
+        if ((rc = do_state_update(c, nullptr, 0))) break;
+        hipEventRecord(ev[6], c->stream);
+        if (hipEventSynchronize(ev[6]) != hipSuccess) {
+            rc = fail(MFCFD_EHIP, "event synchronize");
+            break;
+        }
+        for (int ph = 0; ph < MFCFD_PHASE_COUNT; ++ph) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev[ph], ev[ph + 1]);
+            acc[ph] += ms;
+        }
+        if ((rc = fetch_record(c))) break;
+        residue_bookkeeping(c, t0 + it, c->res_new);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    for (int ph = 0; ph < MFCFD_PHASE_COUNT; ++ph) out_ms[ph] = acc[ph] / n_iters;
+    return MFCFD_OK;
+}
+
 int mfcfd_sweep_info(mfcfd_ctx* c, int* levels_fwd, int* levels_bwd, int* n_swept) {
     if (!c || !c->have_geom) return fail(MFCFD_ESTATE, "no geometry bound");
     if (levels_fwd) *levels_fwd = c->sched.n_levels_fwd;

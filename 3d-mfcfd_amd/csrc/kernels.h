@@ -34,9 +34,10 @@ void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams&
                           hipStream_t st);
 void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
 void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
-// flux_residual.f90
+// flux_residual.f90; class_mask bit 0 interior, bit 1 wall, bit 2 outer
+constexpr int kFluxAll = 7;
 void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p,
-                          const TypeLists& tl, hipStream_t st);
+                          const TypeLists& tl, hipStream_t st, int class_mask = kFluxAll);
 // explicit form of the sweep (delUn = -delt*flux_res, delUp = delUn)
 void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st);
 // implicit_forward_and_reverse_sweeps.f90

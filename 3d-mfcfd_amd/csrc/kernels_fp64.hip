@@ -760,14 +760,14 @@ void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParam
                        g, s, p);
 }
 void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
-                          hipStream_t st) {
-    if (tl.n_wall > 0)
+                          hipStream_t st, int class_mask) {
+    if ((class_mask & 2) && tl.n_wall > 0)
         hipLaunchKernelGGL(k_flux_residual<1>, dim3(grid_for(tl.n_wall, kBlock)), dim3(kBlock), 0, st, g, s,
                            p, tl.wall, tl.n_wall);
-    if (tl.n_outer > 0)
+    if ((class_mask & 4) && tl.n_outer > 0)
         hipLaunchKernelGGL(k_flux_residual<2>, dim3(grid_for(tl.n_outer, kBlock)), dim3(kBlock), 0, st, g,
                            s, p, tl.outer, tl.n_outer);
-    if (tl.n_interior > 0)
+    if ((class_mask & 1) && tl.n_interior > 0)
         hipLaunchKernelGGL(k_flux_residual<0>, dim3(grid_for(tl.n_interior, kBlock)), dim3(kBlock), 0, st,
                            g, s, p, tl.interior, tl.n_interior);
 }

@@ -15,15 +15,15 @@ text decks the reference consumes:
   post_processor.f90:33-39 as ``primal-solution.dat``).
 
 Geometry (all lengths in body units):
-  wall      jittered latitude/longitude grid on the ellipsoid surface with
-            row populations proportional to sin(theta); ``nor`` is the outward
+  wall      jittered latitude/longitude grid on the ellipsoid surface, rows
+            and points spaced uniformly in arc length; ``nor`` is the outward
             body normal (pointing into the fluid), ``tan1`` follows the
             azimuthal direction and ``tan2 = nor x tan1``;
   interior  shells blended from the ellipsoid (t=0) to the far-field sphere
             (t=1); shell offsets grow geometrically from the wall spacing and
-            each shell holds a golden-spiral point set sized so that the
-            tangential spacing matches the radial one; 10 % jitter; Cartesian
-            frame (e_x, e_y, e_z);
+            each shell holds an arc-length latitude/longitude grid sized so the
+            tangential spacing matches the radial one, capped at the far-field
+            point spacing; 10 % jitter; Cartesian frame (e_x, e_y, e_z);
   outer     golden-spiral points on the far-field sphere, ``nor`` radial
             outward, jittered 10 % tangentially.
 
@@ -144,35 +144,78 @@ def _ellipsoid_area(a: float, b: float, c: float) -> float:
     return 4.0 * math.pi * ((a ** p * b ** p + a ** p * c ** p + b ** p * c ** p) / 3.0) ** (1.0 / p)
 
 
-def _wall_points(n_wall: int, axes, rng, jitter):
+def _surface_point(axes, radius, t, th, ph):
+    """Point of the shell surface t in [0,1]: (1-t)*ellipsoid + t*sphere,
+    both at polar angle th and azimuth ph."""
     a, b, c = axes
-    h = math.sqrt(4.0 * math.pi / n_wall)
-    n_rows = max(2, int(round(math.pi / h)))
-    theta_c = (np.arange(n_rows) + 0.5) * math.pi / n_rows
-    w = np.sin(theta_c)
-    raw = w / w.sum() * n_wall
+    th, ph = np.broadcast_arrays(th, ph)
+    t = np.broadcast_to(t, th.shape)[..., None]
+    st, ct, sp, cp = np.sin(th), np.cos(th), np.sin(ph), np.cos(ph)
+    e = np.stack([a * st * cp, b * st * sp, c * ct], axis=-1)
+    d = np.stack([st * cp, st * sp, ct], axis=-1)
+    return (1.0 - t) * e + t * radius * d
+
+
+def _latlong_on_surface(m, axes, radius, t, rng, jitter):
+    """m points in a jittered latitude/longitude grid that is uniform in
+    arc length on the shell surface t: rows equally spaced along the
+    (azimuth-averaged) meridian, points equally spaced along each row.
+    Returns the points and their (theta, phi) parameters."""
+    if m <= 0:
+        return np.zeros((0, 3)), np.zeros(0), np.zeros(0)
+    th_f = np.linspace(0.0, math.pi, 2049)
+    ph_s = np.linspace(0.0, 2.0 * math.pi, 16, endpoint=False)
+    pts = _surface_point(axes, radius, t, th_f[:, None], ph_s[None, :])
+    seg = np.linalg.norm(np.diff(pts, axis=0), axis=-1).mean(axis=1)
+    s_m = np.concatenate([[0.0], np.cumsum(seg)])
+    length = s_m[-1]
+    # area ~ sum over rows of perimeter * ds; pick the spacing h that gives m
+    th_mid = 0.5 * (th_f[1:] + th_f[:-1])
+    ph_f = np.linspace(0.0, 2.0 * math.pi, 257)
+    ring = _surface_point(axes, radius, t, th_mid[::16, None], ph_f[None, :])
+    perim_c = np.linalg.norm(np.diff(ring, axis=1), axis=-1).sum(axis=1)
+    area = np.sum(np.interp(th_mid, th_mid[::16], perim_c) * seg)
+    h = math.sqrt(area / m)
+    n_rows = max(1, int(round(length / h)))
+    s_rows = (np.arange(n_rows) + 0.5) * length / n_rows
+    th_rows = np.interp(s_rows, s_m, th_f)
+    ring = _surface_point(axes, radius, t, th_rows[:, None], ph_f[None, :])
+    seglen = np.linalg.norm(np.diff(ring, axis=1), axis=-1)
+    perim = seglen.sum(axis=1)
+    raw = perim / perim.sum() * m
     cnt = np.maximum(1, np.floor(raw).astype(np.int64))
-    # largest-remainder rounding to hit n_wall exactly
-    while cnt.sum() < n_wall:
+    while cnt.sum() < m:
         cnt[np.argmax(raw - cnt)] += 1
-    while cnt.sum() > n_wall:
+    while cnt.sum() > m:
         k = np.argmax(np.where(cnt > 1, cnt - raw, -np.inf))
         cnt[k] -= 1
-    thetas, phis = [], []
     dth = math.pi / n_rows
+    ths, phs = [], []
     for r in range(n_rows):
-        m = int(cnt[r])
-        dph = 2.0 * math.pi / m
-        ph = (np.arange(m) + 0.5) * dph + rng.uniform(0.0, dph)
-        ph = ph + rng.uniform(-jitter, jitter, size=m) * dph
-        th = theta_c[r] + rng.uniform(-jitter, jitter, size=m) * dth
-        thetas.append(th)
-        phis.append(ph)
-    th = np.concatenate(thetas)
-    ph = np.concatenate(phis)
+        k = int(cnt[r])
+        cum = np.concatenate([[0.0], np.cumsum(seglen[r])])
+        u = (np.arange(k) + 0.5 + rng.uniform(-jitter, jitter, size=k)) / k
+        u = (u + rng.uniform(0.0, 1.0 / k)) % 1.0
+        ph = np.interp(u * perim[r], cum, ph_f)
+        # jitter the row position by a fraction of the local row spacing
+        j = rng.uniform(-jitter, jitter, size=k)
+        lo = th_rows[max(r - 1, 0)] if r > 0 else 0.0
+        hi = th_rows[min(r + 1, n_rows - 1)] if r < n_rows - 1 else math.pi
+        th = th_rows[r] + j * (0.5 * (hi - lo) if n_rows > 1 else dth)
+        ths.append(np.clip(th, 1e-9, math.pi - 1e-9))
+        phs.append(ph)
+    th = np.concatenate(ths)
+    ph = np.concatenate(phs)
+    return _surface_point(axes, radius, t, th, ph), th, ph
+
+
+def _wall_points(n_wall: int, axes, rng, jitter):
+    """Jittered latitude/longitude grid on the ellipsoid, uniform in arc
+    length so that the thin rim is sampled as finely as the faces."""
+    a, b, c = axes
+    p, th, ph = _latlong_on_surface(n_wall, axes, 1.0, 0.0, rng, jitter)
     st, ct = np.sin(th), np.cos(th)
     sp, cp = np.sin(ph), np.cos(ph)
-    p = np.stack([a * st * cp, b * st * sp, c * ct], axis=1)
     nor = _normalize(np.stack([st * cp / a, st * sp / b, ct / c], axis=1))
     hint = np.stack([-a * sp, b * cp, np.zeros_like(sp)], axis=1)
     t1, t2 = _frame_from_normal(nor, hint)
@@ -180,20 +223,23 @@ def _wall_points(n_wall: int, axes, rng, jitter):
 
 
 def _shell_plan(n_interior: int, h_wall: float, rho_e: float, radius: float,
-                aspect: float = 1.0):
+                aspect: float = 1.0, h_cap: float = float("inf")):
     """Shell offsets (distance from the wall) and point counts.
 
-    Offsets follow delta_s = h_w*(g^s - 1)/(g - 1); the growth g is chosen by
-    bisection so that shells holding aspect*4*pi*rho^2/h^2 points each (aspect
-    1: tangential spacing equal to the radial one) sum to n_interior; the
-    shells stop half a spacing short of the far field.
+    Shell spacing grows geometrically from the wall spacing, h_s = h_w*g^s,
+    until it reaches h_cap (the far-field point spacing) and stays there, so
+    the last shells match the far-field points and every outer point sees
+    interior neighbours as well as far-field ones.  The growth g is chosen
+    by bisection so that shells holding aspect*4*pi*rho^2/h^2 points each
+    (aspect 1: tangential spacing equal to the radial one) sum to
+    n_interior; the shells stop half a spacing short of the far field.
     """
     span = radius - rho_e
 
     def plan(g):
         deltas, hs = [], []
         d = 0.0
-        h = h_wall
+        h = min(h_wall, h_cap)
         while True:
             d_next = d + h
             if d_next > span - 0.5 * h:
@@ -201,7 +247,7 @@ def _shell_plan(n_interior: int, h_wall: float, rho_e: float, radius: float,
             deltas.append(d_next)
             hs.append(h)
             d = d_next
-            h *= g
+            h = min(h * g, h_cap)
         deltas = np.array(deltas)
         hs = np.array(hs)
         rho = rho_e + deltas
@@ -226,21 +272,22 @@ def _shell_plan(n_interior: int, h_wall: float, rho_e: float, radius: float,
     return deltas, hs, cnt
 
 
-def _interior_points(n_interior, axes, radius, h_wall, rng, jitter, aspect, radial_jitter):
+def _interior_points(n_interior, axes, radius, h_wall, rng, jitter, aspect, radial_jitter,
+                     h_cap):
     a, b, c = axes
     rho_e = math.sqrt(_ellipsoid_area(a, b, c) / (4.0 * math.pi))
-    deltas, hs, cnt = _shell_plan(n_interior, h_wall, rho_e, radius, aspect)
+    deltas, hs, cnt = _shell_plan(n_interior, h_wall, rho_e, radius, aspect, h_cap)
     span = radius - rho_e
     pts = []
     for d, h, m in zip(deltas, hs, cnt):
         m = int(m)
         if m == 0:
             continue
-        dirs = _golden_directions(m, rng, jitter)
+        t0 = d / span
+        _, th, ph = _latlong_on_surface(m, axes, radius, t0, rng, jitter)
         t = (d + rng.uniform(-radial_jitter, radial_jitter, size=m) * h) / span
-        t = np.clip(t, 1e-6, 1.0 - 1e-6)[:, None]
-        e = dirs * np.array([a, b, c])
-        pts.append((1.0 - t) * e + t * radius * dirs)
+        t = np.clip(t, 1e-6, 1.0 - 1e-6)
+        pts.append(_surface_point(axes, radius, t, th, ph))
     p = np.concatenate(pts, axis=0) if pts else np.zeros((0, 3))
     return p, len(deltas)
 
@@ -315,43 +362,92 @@ def _knn(p: np.ndarray, k: int, workers: int):
     return nb.astype(np.int64), nd
 
 
-def _split_counts(p, nb, rows):
+def _split_counts(p, nb, rows, chunk=262144):
     """Per-row counts of (del <= 0, del > 0) along x, y, z (Cartesian frame)."""
-    d = p[nb[rows]] - p[rows][:, None, :]
-    le = (d <= 0.0).sum(axis=1)
-    gt = (d > 0.0).sum(axis=1)
+    le = np.empty((rows.size, 3), dtype=np.int32)
+    gt = np.empty((rows.size, 3), dtype=np.int32)
+    for s0 in range(0, rows.size, chunk):
+        r = rows[s0:s0 + chunk]
+        d = p[nb[r]] - p[r][:, None, :]
+        c = (d <= 0.0).sum(axis=1)
+        le[s0:s0 + chunk] = c
+        gt[s0:s0 + chunk] = nb.shape[1] - c
     return le, gt
 
 
-def _repair_interior(p, status, axes, radius, k, workers, min_split, max_passes=12):
+def _repair_interior(p, status, axes, radius, k, workers, min_split, max_passes=40,
+                     seed=0):
     """Nudge interior points whose Cartesian half-stencils are too small.
 
     A half-stencil with fewer than 3 points makes the 3x3 least-squares
     system singular (and the reference then divides by a zero determinant).
     Such points sit where a curved shell puts every same-shell neighbour on
-    one side; moving the point a fraction of its nearest-neighbour distance
-    toward the starved side and recomputing the k nearest neighbours fixes
-    them.  Points are kept outside the body and inside the far field.
+    one side; moving the point toward the starved side by a growing fraction
+    of its nearest-neighbour distance (plus a small random component that
+    breaks ties between passes) fixes them.  After each pass only the rows
+    whose k-nearest lists can have changed are queried again.  Points are
+    kept outside the body and inside the far field.
     """
+    from scipy.spatial import cKDTree
     a, b, c = axes
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
     interior = np.flatnonzero(status == STATUS_INTERIOR)
+    is_interior = status == STATUS_INTERIOR
     nb, nd = _knn(p, k, workers)
-    for _ in range(max_passes):
-        le, gt = _split_counts(p, nb, interior)
+    row_of = np.full(p.shape[0], -1, dtype=np.int64)
+    row_of[interior] = np.arange(interior.size)
+    le, gt = _split_counts(p, nb, interior)
+    for it in range(max_passes):
         bad = (le < min_split) | (gt < min_split)
         rows = np.flatnonzero(bad.any(axis=1))
         if rows.size == 0:
             return nb, nd, 0
         idx = interior[rows]
-        step = 0.35 * nd[idx, 0][:, None]
+        frac = 0.3 + 0.05 * it
+        step = frac * nd[idx, 0][:, None]
         move = np.where(le[rows] < min_split, 1.0, 0.0) - np.where(gt[rows] < min_split, 1.0, 0.0)
+        move = move + rng.uniform(-0.3, 0.3, size=move.shape)
         newp = p[idx] + step * move
+        # a move blocked by the body or the far field pulls the point halfway
+        # toward the centroid of its neighbours instead
+        inside = (newp[:, 0] / a) ** 2 + (newp[:, 1] / b) ** 2 + (newp[:, 2] / c) ** 2 <= 1.0
+        outside = np.linalg.norm(newp, axis=1) >= radius
+        blocked = inside | outside
+        if blocked.any():
+            cen = p[nb[idx[blocked]]].mean(axis=1)
+            newp[blocked] = 0.5 * p[idx[blocked]] + 0.5 * cen
         inside = (newp[:, 0] / a) ** 2 + (newp[:, 1] / b) ** 2 + (newp[:, 2] / c) ** 2 <= 1.0
         outside = np.linalg.norm(newp, axis=1) >= radius
         ok = ~(inside | outside)
-        p[idx[ok]] = newp[ok]
-        nb, nd = _knn(p, k, workers)
-    le, gt = _split_counts(p, nb, interior)
+        moved = idx[ok]
+        if moved.size == 0:
+            break
+        old = p[moved].copy()
+        p[moved] = newp[ok]
+        # rows to refresh: the moved points, every row that listed one of
+        # them, and every row close enough to list one at its new position
+        tree = cKDTree(p)
+        reach = nd[moved, -1] * 1.5 + np.linalg.norm(p[moved] - old, axis=1)
+        near = tree.query_ball_point(p[moved], reach, workers=workers)
+        touched = np.zeros(p.shape[0], dtype=bool)
+        touched[moved] = True
+        for lst in near:
+            touched[lst] = True
+        touched |= np.isin(nb, moved).any(axis=1)
+        rows_q = np.flatnonzero(touched)
+        dist, ind = tree.query(p[rows_q], k=k + 1, workers=workers)
+        self_col = ind == rows_q[:, None]
+        has_self = self_col.any(axis=1)
+        keep = ~self_col
+        keep[~has_self, -1] = False
+        nb[rows_q] = ind[keep].reshape(-1, k)
+        nd[rows_q] = dist[keep].reshape(-1, k)
+        # only rows whose point or neighbour list changed need new counts
+        ri = row_of[rows_q]
+        sel = ri >= 0
+        le_q, gt_q = _split_counts(p, nb, rows_q[sel])
+        le[ri[sel]] = le_q
+        gt[ri[sel]] = gt_q
     left = int(((le < min_split) | (gt < min_split)).any(axis=1).sum())
     return nb, nd, left
 
@@ -391,8 +487,9 @@ def ellipsoid_cloud(n_wall: int, n_outer: int, n_interior: int, k: int = 25,
     h_wall = math.sqrt(_ellipsoid_area(a, b, c) / n_wall)
     rj = jitter if radial_jitter is None else radial_jitter
     if interior == "shells":
+        h_cap = math.sqrt(4.0 * math.pi * radius * radius / n_outer)
         pi_, n_shells = _interior_points(n_interior, axes, radius, h_wall, rng, jitter,
-                                         shell_aspect, rj)
+                                         shell_aspect, rj, h_cap)
     elif interior == "lattice":
         pi_, _h = _lattice_interior(n_interior, axes, radius, rng, jitter)
         n_shells = 0
@@ -412,7 +509,8 @@ def ellipsoid_cloud(n_wall: int, n_outer: int, n_interior: int, k: int = 25,
         raise ValueError("k must be smaller than the number of points")
 
     if min_split > 0:
-        nb, nd, left = _repair_interior(p, status, axes, radius, k, workers, min_split)
+        nb, nd, left = _repair_interior(p, status, axes, radius, k, workers, min_split,
+                                        seed=seed)
         if left:
             raise RuntimeError(f"{left} interior points keep a half-stencil below {min_split}")
     else:

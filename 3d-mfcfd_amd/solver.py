@@ -66,7 +66,10 @@ EXPORTS = ["mfcfd_create", "mfcfd_destroy", "mfcfd_last_error", "mfcfd_abi_versi
            "mfcfd_compute_conserved_vector", "mfcfd_eval_q_variables",
            "mfcfd_eval_q_derivatives", "mfcfd_timestep_delt", "mfcfd_eval_flux_residual",
            "mfcfd_implicit_forward_backward_sweeps", "mfcfd_state_update", "mfcfd_fpi_solver",
-           "mfcfd_run", "mfcfd_get_residual", "mfcfd_synchronize", "mfcfd_sweep_info"]
+           "mfcfd_run", "mfcfd_get_residual", "mfcfd_synchronize", "mfcfd_sweep_info",
+           "mfcfd_time_phases"]
+PHASES = ["conserved_q", "qderiv_timestep", "flux_interior", "flux_boundary", "sweeps",
+          "state_update"]
 
 
 def lib():
@@ -96,6 +99,7 @@ def lib():
         L.mfcfd_run.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         L.mfcfd_get_residual.argtypes = [vp, C.POINTER(Residual)]
         L.mfcfd_sweep_info.argtypes = [vp, _ip, _ip, _ip]
+        L.mfcfd_time_phases.argtypes = [vp, C.c_int, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -252,6 +256,13 @@ class Solver:
 
     def synchronize(self):
         _check(lib().mfcfd_synchronize(self._ctx))
+
+    def time_phases(self, t0, n_iters):
+        """Mean milliseconds per iteration of each phase (HIP events on the
+        context's stream); advances the state by n_iters iterations."""
+        out = np.zeros(len(PHASES))
+        _check(lib().mfcfd_time_phases(self._ctx, t0, n_iters, _d(out)))
+        return dict(zip(PHASES, out.tolist()))
 
     def sweep_info(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -143,6 +143,19 @@ int mfcfd_run(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, double 
 int mfcfd_get_residual(mfcfd_ctx *ctx, mfcfd_residual *out);
 int mfcfd_synchronize(mfcfd_ctx *ctx);
 
+/* diagnostics: run n_iters iterations starting at t0 (advancing the state
+ * exactly as mfcfd_run does) with HIP events recorded on the context's
+ * stream between phases; out_ms[MFCFD_PHASE_*] receives the mean
+ * milliseconds per iteration of each phase */
+#define MFCFD_PHASE_CONSERVED_Q 0    /* compute_conserved_vector + eval_q_variables */
+#define MFCFD_PHASE_QDERIV_DT 1      /* eval_q_derivatives + timestep_delt */
+#define MFCFD_PHASE_FLUX_INTERIOR 2  /* eval_flux_residual, interior points */
+#define MFCFD_PHASE_FLUX_BOUNDARY 3  /* eval_flux_residual, wall and outer points */
+#define MFCFD_PHASE_SWEEPS 4         /* implicit_forward_backward_sweeps (or explicit form) */
+#define MFCFD_PHASE_STATE_UPDATE 5   /* state_update + residual */
+#define MFCFD_PHASE_COUNT 6
+int mfcfd_time_phases(mfcfd_ctx *ctx, int t0, int n_iters, double *out_ms);
+
 /* diagnostics: sweep schedule depth (levels of the forward and backward
  * Gauss-Seidel dependency chains) and swept point count */
 int mfcfd_sweep_info(mfcfd_ctx *ctx, int *levels_fwd, int *levels_bwd, int *n_swept);

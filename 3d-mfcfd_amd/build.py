@@ -28,6 +28,7 @@ NO_PACKED_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 UNITS = {
     "capi.hip": ["-ffp-contract=off"],
     "kernels_fp64.hip": ["-ffp-contract=off"],
+    "kernels_flux_fused.hip": ["-ffp-contract=off"],
     "preprocess.cpp": ["-ffp-contract=off", "-x", "hip"],
 }
 

@@ -1,17 +1,22 @@
 // capi.hip -- C ABI of libmfcfd_hip.so (declared in include/mfcfd_hip.h).
 //
-// A context owns the HBM copies of the point cloud and of the flow state in
-// the reference's layouts, the class index lists, the sweep schedule and
-// the residual buffers.  Entry points enqueue kernels on the context's
-// stream and return; only the fpi_solver/run/get_residual/download calls
-// wait for the device.
+// A context owns the HBM copies of the point cloud and the flow state.  At
+// bind time the points are renumbered in Morton order (device order), each
+// point's six direction lists are resolved from the host's (22,N) arrays
+// exactly as the reference reads them, regular points are marked for the
+// fused flux kernel, and the Cramer weights of every list are computed.
+// Entry points enqueue kernels on the context's stream and return; only
+// fpi_solver/run/state_update/get_residual/download wait for the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -61,6 +66,44 @@ struct DevBuf {
     }
 };
 
+// 63-bit Morton code of a point in the cloud's bounding box, 21 bits per axis
+uint64_t spread21(uint64_t v) {
+    v &= 0x1FFFFFULL;
+    v = (v | (v << 32)) & 0x1F00000000FFFFULL;
+    v = (v | (v << 16)) & 0x1F0000FF0000FFULL;
+    v = (v | (v << 8)) & 0x100F00F00F00F00FULL;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+std::vector<int> morton_order(int n, const double* x, const double* y, const double* z) {
+    double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+    for (int i = 0; i < n; ++i) {
+        const double v[3] = {x[i], y[i], z[i]};
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], v[c]);
+            hi[c] = std::max(hi[c], v[c]);
+        }
+    }
+    std::vector<std::pair<uint64_t, int>> key(n);
+    for (int i = 0; i < n; ++i) {
+        const double v[3] = {x[i], y[i], z[i]};
+        uint64_t code = 0;
+        for (int c = 0; c < 3; ++c) {
+            const double span = hi[c] > lo[c] ? hi[c] - lo[c] : 1.0;
+            double t = (v[c] - lo[c]) / span * 2097151.0;
+            t = std::min(std::max(t, 0.0), 2097151.0);
+            code |= spread21((uint64_t)t) << c;
+        }
+        key[i] = {code, i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm(n);
+    for (int d = 0; d < n; ++d) perm[d] = key[d].second;
+    return perm;
+}
+
 }  // namespace
 
 struct mfcfd_ctx {
@@ -72,14 +115,17 @@ struct mfcfd_ctx {
     FlowParams fp{};
 
     int n = 0;
-    // geometry
+    // geometry, device order
     DevBuf<double4> pos;
-    DevBuf<double> frame;
-    DevBuf<int> status, nbhs, conn, split_cnt, split, alias, respos;
+    DevBuf<double> frame, coef;
+    DevBuf<int> status, nbhs, conn, dir_cnt, lst, alias, respos, perm;
+    DevBuf<long long> dir_off;
+    DevBuf<unsigned char> mask;
     DevBuf<int> idx_interior, idx_wall, idx_outer, idx_sup_in, idx_sup_out;
+    DevBuf<int> idx_fused_interior, idx_fused_wall, idx_generic;
     TypeLists tl{};
     // state
-    DevBuf<double> prim, U, q, qm, dq, delt, flux_res, delUp, delUn, dinv, temp;
+    DevBuf<double> prim, U, q, qm, dq, delt, flux_res, delUp, delUn, dinv, temp, staging;
     // residual
     DevBuf<double> partial;
     DevBuf<int> partial_pos;
@@ -87,13 +133,15 @@ struct mfcfd_ctx {
     DevBuf<ResidualRecord> rec;
     DevBuf<double> hist;
     ResidualRecord* host_rec = nullptr;  // pinned
-    // sweep schedule
+    // sweep schedule (device point numbers)
     Schedule sched;
+    std::vector<int> fwd_ptr, bwd_ptr;
     DevBuf<int> fwd_points, bwd_points;
     bool implicit_supported = true;
     // bookkeeping of fpi_solver.f90:38-43
     double res_old = 0.0, res_new = 0.0, residue = 0.0, max_res = 0.0, sum_res_sqr = 0.0;
     int max_res_point = 0, last_t = 0;
+    int n_regular = 0;
 
     DevGeom geom() const {
         DevGeom g;
@@ -103,11 +151,14 @@ struct mfcfd_ctx {
         g.status = status.p;
         g.nbhs = nbhs.p;
         g.conn = conn.p;
-        g.split_cnt = split_cnt.p;
-        g.split = split.p;
-        g.split_stride = 22LL * n + kSplitSlack;
+        g.dir_cnt = dir_cnt.p;
+        g.dir_off = dir_off.p;
+        g.lst = lst.p;
+        g.mask = mask.p;
+        g.coef = coef.p;
         g.alias = alias.p;
         g.respos = respos.p;
+        g.perm = perm.p;
         return g;
     }
     DevState state() const {
@@ -124,15 +175,20 @@ struct mfcfd_ctx {
         s.dinv = dinv.p;
         return s;
     }
-    ~mfcfd_ctx() {
-        DevBuf<double>* dbl[] = {&frame, &prim, &U, &q, &qm, &dq, &delt, &flux_res, &delUp, &delUn, &dinv,
-                                 &temp, &partial, &hist};
+    void release_all() {
+        DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &q, &qm, &dq, &delt, &flux_res, &delUp, &delUn,
+                                 &dinv, &temp, &staging, &partial, &hist};
         for (auto* b : dbl) b->release();
-        DevBuf<int>* ints[] = {&status, &nbhs, &conn, &split_cnt, &split, &alias, &respos, &idx_interior,
-                               &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &partial_pos,
-                               &fwd_points, &bwd_points};
+        DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &idx_interior,
+                               &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
+                               &idx_fused_wall, &idx_generic, &partial_pos, &fwd_points, &bwd_points};
         for (auto* b : ints) b->release();
+        dir_off.release();
+        mask.release();
         pos.release();
+    }
+    ~mfcfd_ctx() {
+        release_all();
         rec.release();
         if (host_rec) (void)hipHostFree(host_rec);
         if (stream) (void)hipStreamDestroy(stream);
@@ -158,7 +214,7 @@ int upload(DevBuf<T>& b, const T* host, size_t count, hipStream_t st) {
 
 struct ArrayDesc {
     DevBuf<double>* buf;
-    size_t per_point;
+    int per_point;
 };
 
 ArrayDesc array_of(mfcfd_ctx* c, int id) {
@@ -180,6 +236,13 @@ int launch_error(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(MFCFD_EHIP, "%s launch: %s", what, hipGetErrorString(e));
     return MFCFD_OK;
+}
+
+void flux(mfcfd_ctx* c, int parts) {
+    if (c->params.flux_order == MFCFD_FLUX_LISTS)
+        launch_flux_residual_reference(c->geom(), c->state(), c->fp, c->tl, c->stream);
+    else
+        launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream, parts);
 }
 
 int do_sweeps(mfcfd_ctx* c) {
@@ -220,7 +283,7 @@ int do_iteration(mfcfd_ctx* c, double* hist, int hist_index) {
     } else {
         launch_qderiv_timestep(g, s, c->fp, c->stream);
     }
-    launch_flux_residual(g, s, c->fp, c->tl, c->stream);
+    flux(c, kFluxAll);
     int rc = launch_error("iteration");
     if (rc) return rc;
     if ((rc = do_sweeps(c))) return rc;
@@ -247,6 +310,23 @@ int fetch_record(mfcfd_ctx* c) {
     c->max_res = c->host_rec->max_res;
     if (c->host_rec->max_res_point > 0) c->max_res_point = c->host_rec->max_res_point;
     return MFCFD_OK;
+}
+
+// The list direction d of host point r walks, as the reference reads it:
+// split lists through the raw (22,N) offset, wall zneg and outer zneg
+// replaced by what the reference walks instead (conn / nothing).
+struct HostList {
+    const int* p;
+    int count;
+};
+
+HostList host_list(const mfcfd_geometry* gm, int r, int d) {
+    const int st = gm->status[r];
+    if (st == 1 && d == ZNEG) return {gm->conn + 27LL * r, gm->nbhs[r]};  // wall_flux_dGzneg.f90:61-65
+    if (st == 1 && d == ZPOS) return {nullptr, 0};                          // not walked for wall points
+    if (st == 2 && d == ZNEG) return {nullptr, 0};                          // not walked for outer points
+    if (st != 0 && st != 1 && st != 2) return {nullptr, 0};
+    return {gm->split_conn[d] + 22LL * r, gm->split_nbhs[d][r]};
 }
 
 }  // namespace
@@ -294,8 +374,12 @@ int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
     if (p->precision != MFCFD_FP64 && p->precision != MFCFD_MIXED)
         return fail(MFCFD_EINVAL, "precision %d", p->precision);
     if (p->precision == MFCFD_MIXED) return fail(MFCFD_EUNSUPPORTED, "mixed precision is not built in");
+    if (p->flux_order != MFCFD_FLUX_FUSED && p->flux_order != MFCFD_FLUX_LISTS)
+        return fail(MFCFD_EINVAL, "flux_order %d", p->flux_order);
     if (!(p->cfl > 0.0)) return fail(MFCFD_EINVAL, "CFL must be positive");
     if (p->inner_iterations < 0) return fail(MFCFD_EINVAL, "inner_iterations < 0");
+    if (c->have_geom && p->power != c->params.power)
+        return fail(MFCFD_EINVAL, "power changes the stencil weights; set it before mfcfd_bind_geometry");
     c->params = *p;
     // parameter.f90:13-33
     FlowParams& f = c->fp;
@@ -358,13 +442,11 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
     for (int d = 0; d < 6; ++d)
         if (!gm->split_nbhs[d] || !gm->split_conn[d]) return fail(MFCFD_EINVAL, "null split array %d", d);
 
-    // validate: neighbour numbers, list lengths and the last point's spill
-    std::vector<int> cls_count(7, 0);
+    // ---- validation --------------------------------------------------
     for (int i = 0; i < n; ++i) {
         const int st = gm->status[i];
         if (st != 0 && st != 1 && st != 2 && st != 5 && st != 6)
             return fail(MFCFD_EINVAL, "point %d: unknown status %d", i + 1, st);
-        cls_count[st]++;
         if (gm->nbhs[i] < 0 || gm->nbhs[i] > kConnCols)
             return fail(MFCFD_EINVAL, "point %d: nbhs=%d outside 0..27", i + 1, gm->nbhs[i]);
         for (int j = 0; j < gm->nbhs[i]; ++j) {
@@ -372,88 +454,148 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
             if (k < 1 || k > n) return fail(MFCFD_EINVAL, "point %d: conn(%d)=%d out of range", i + 1, j + 1, k);
         }
     }
-    for (int d = 0; d < 6; ++d) {
-        for (int i = 0; i < n; ++i) {
-            const int cnt = gm->split_nbhs[d][i];
-            if (cnt < 0 || cnt > kConnCols)
-                return fail(MFCFD_EINVAL, "point %d: split list %d has %d entries", i + 1, d, cnt);
-        }
-        if (gm->split_nbhs[d][n - 1] > kSplitCols && !(d == 4 && gm->status[n - 1] == 1) &&
-            !(d == 5 && gm->status[n - 1] == 2))
-            return fail(MFCFD_EINVAL,
-                        "the last point's split list %d has %d > 22 entries: the reference stores them "
-                        "past the end of its (22,N) array",
-                        d, gm->split_nbhs[d][n - 1]);
-    }
-    // lists that are read: entries must be valid point numbers
-    auto list_used = [&](int d, int i) {
-        const int st = gm->status[i];
-        if (st == 0) return true;
-        if (st == 1) return d <= 3;
-        if (st == 2) return d <= 4;
-        return false;
-    };
     for (int d = 0; d < 6; ++d)
         for (int i = 0; i < n; ++i) {
-            if (!list_used(d, i)) continue;
-            for (int j = 0; j < gm->split_nbhs[d][i]; ++j) {
-                const long long off = 22LL * i + j;
-                if (off >= 22LL * n) return fail(MFCFD_EINVAL, "split list %d of point %d runs past the array", d, i + 1);
-                const int k = gm->split_conn[d][off];
-                if (k < 1 || k > n)
-                    return fail(MFCFD_EINVAL, "point %d: split list %d entry %d = %d out of range", i + 1, d, j + 1, k);
+            const HostList hl = host_list(gm, i, d);
+            if (!hl.p) continue;
+            if (hl.count < 0 || hl.count > kConnCols)
+                return fail(MFCFD_EINVAL, "point %d: list %d has %d entries", i + 1, d, hl.count);
+            if (hl.p != gm->conn + 27LL * i && 22LL * i + hl.count > 22LL * n)
+                return fail(MFCFD_EINVAL,
+                            "point %d: split list %d has %d entries and runs past the (22,N) array "
+                            "(the reference writes and reads it out of bounds)",
+                            i + 1, d, hl.count);
+            for (int j = 0; j < hl.count; ++j)
+                if (hl.p[j] < 1 || hl.p[j] > n)
+                    return fail(MFCFD_EINVAL, "point %d: list %d entry %d = %d out of range", i + 1, d, j + 1,
+                                hl.p[j]);
+        }
+    for (int i = 0; i < n; ++i) {
+        const int p = gm->point_with_alias[i];
+        if (p < 0 || p > n) return fail(MFCFD_EINVAL, "point_with_alias(%d)=%d out of range", i + 1, p);
+    }
+
+    // ---- device order ------------------------------------------------
+    std::vector<int> perm = morton_order(n, gm->x, gm->y, gm->z);  // device -> host
+    std::vector<int> inv(n);
+    for (int d = 0; d < n; ++d) inv[perm[d]] = d;
+
+    std::vector<double4> pos4(n);
+    std::vector<double> fr(9ULL * n);
+    std::vector<int> status(n), nbhs(n), conn0(27ULL * n, -1), alias(n);
+    for (int d = 0; d < n; ++d) {
+        const int r = perm[d];
+        pos4[d] = make_double4(gm->x[r], gm->y[r], gm->z[r], gm->min_dist[r]);
+        for (int cc = 0; cc < 3; ++cc) {
+            fr[9ULL * d + cc] = gm->tan1[3ULL * r + cc];
+            fr[9ULL * d + 3 + cc] = gm->tan2[3ULL * r + cc];
+            fr[9ULL * d + 6 + cc] = gm->nor[3ULL * r + cc];
+        }
+        status[d] = gm->status[r];
+        nbhs[d] = gm->nbhs[r];
+        for (int j = 0; j < nbhs[d]; ++j) conn0[27ULL * d + j] = inv[gm->conn[27LL * r + j] - 1];
+        alias[d] = gm->alias[r];
+    }
+
+    // direction lists, masks and regularity
+    std::vector<int> cnt(6ULL * n, 0);
+    std::vector<long long> off(6ULL * n, 0);
+    std::vector<int> lst;
+    lst.reserve((size_t)n * 76);
+    std::vector<unsigned char> mask(27ULL * n, 0);
+    std::vector<unsigned char> regular(n, 0);
+    for (int d = 0; d < n; ++d) {
+        const int r = perm[d];
+        const int st = status[d];
+        bool reg = (st == 0 || st == 1);
+        const int nb = nbhs[d];
+        const int* cr = gm->conn + 27LL * r;
+        for (int dir = 0; dir < 6; ++dir) {
+            const HostList hl = host_list(gm, r, dir);
+            off[(size_t)dir * n + d] = (long long)lst.size();
+            cnt[(size_t)dir * n + d] = hl.p ? hl.count : 0;
+            if (!hl.p) continue;
+            // membership: an in-order subsequence of conn sets mask bits
+            int j = 0;
+            for (int e = 0; e < hl.count; ++e) {
+                const int k1 = hl.p[e];
+                lst.push_back(inv[k1 - 1]);
+                while (j < nb && cr[j] != k1) ++j;
+                if (j < nb) {
+                    mask[27ULL * d + j] |= (unsigned char)(1u << dir);
+                    ++j;
+                } else {
+                    reg = false;
+                }
             }
         }
+        if (reg) {
+            // interior: each axis partitions conn; wall: x/y cover conn and
+            // zneg is conn itself
+            for (int j = 0; j < nb && reg; ++j) {
+                const unsigned b = mask[27ULL * d + j];
+                for (int a = 0; a < 3; ++a) {
+                    const unsigned ab = (b >> (2 * a)) & 3u;
+                    if (st == 0 && !(ab == 1u || ab == 2u)) reg = false;
+                    if (st == 1 && a < 2 && ab == 0u) reg = false;
+                }
+                if (st == 1 && !(b & (1u << ZNEG))) reg = false;
+            }
+        }
+        regular[d] = reg ? 1 : 0;
+    }
+    c->n_regular = 0;
+    for (int d = 0; d < n; ++d) c->n_regular += regular[d];
 
+    // class lists (device numbers, device order) and the residual order
+    std::vector<int> li, lw, lo, lsi, lso, lfi, lfw, lgen;
+    for (int d = 0; d < n; ++d) {
+        switch (status[d]) {
+            case 0: li.push_back(d); (regular[d] ? lfi : lgen).push_back(d); break;
+            case 1: lw.push_back(d); (regular[d] ? lfw : lgen).push_back(d); break;
+            case 2: lo.push_back(d); lgen.push_back(d); break;
+            case 5: lsi.push_back(d); break;
+            case 6: lso.push_back(d); break;
+        }
+    }
+    // the reference visits wall points, then interior points, each in host order
+    std::vector<int> respos(n, 0x7fffffff);
+    {
+        int pos_w = 0, pos_i = 0, nw = 0;
+        for (int r = 0; r < n; ++r) nw += gm->status[r] == 1;
+        for (int r = 0; r < n; ++r) {
+            if (gm->status[r] == 1) respos[inv[r]] = pos_w++;
+            else if (gm->status[r] == 0) respos[inv[r]] = nw + pos_i++;
+        }
+        // supersonic points are walked in host order
+        std::sort(lsi.begin(), lsi.end(), [&](int a, int b) { return perm[a] < perm[b]; });
+        std::sort(lso.begin(), lso.end(), [&](int a, int b) { return perm[a] < perm[b]; });
+    }
+
+    c->release_all();
+    c->have_geom = false;
     c->n = n;
     hipStream_t st = c->stream;
-    // positions with min_dist packed in .w
-    std::vector<double4> pos4(n);
-    for (int i = 0; i < n; ++i) pos4[i] = make_double4(gm->x[i], gm->y[i], gm->z[i], gm->min_dist[i]);
-    std::vector<double> fr(9ULL * n);
-    for (int i = 0; i < n; ++i)
-        for (int cc = 0; cc < 3; ++cc) {
-            fr[9ULL * i + cc] = gm->tan1[3ULL * i + cc];
-            fr[9ULL * i + 3 + cc] = gm->tan2[3ULL * i + cc];
-            fr[9ULL * i + 6 + cc] = gm->nor[3ULL * i + cc];
-        }
-    std::vector<int> conn0(27ULL * n);
-    for (size_t e = 0; e < conn0.size(); ++e) conn0[e] = gm->conn[e] - 1;
-    const long long stride = 22LL * n + kSplitSlack;
-    std::vector<int> cnt(6ULL * n), split0(6ULL * stride, -1);
-    for (int d = 0; d < 6; ++d) {
-        std::memcpy(&cnt[(size_t)d * n], gm->split_nbhs[d], sizeof(int) * (size_t)n);
-        for (long long e = 0; e < 22LL * n; ++e) split0[d * stride + e] = gm->split_conn[d][e] - 1;
-    }
-    // residual order: wall points, then interior points (state_update.f90:29,62)
-    std::vector<int> respos(n, 0x7fffffff);
-    std::vector<int> li, lw, lo, lsi, lso;
-    for (int i = 0; i < n; ++i) {
-        switch (gm->status[i]) {
-            case 0: li.push_back(i); break;
-            case 1: lw.push_back(i); break;
-            case 2: lo.push_back(i); break;
-            case 5: lsi.push_back(i); break;
-            case 6: lso.push_back(i); break;
-        }
-    }
-    for (size_t m = 0; m < lw.size(); ++m) respos[lw[m]] = (int)m;
-    for (size_t m = 0; m < li.size(); ++m) respos[li[m]] = (int)(lw.size() + m);
-
     if ((rc = upload(c->pos, pos4.data(), pos4.size(), st))) return rc;
     if ((rc = upload(c->frame, fr.data(), fr.size(), st))) return rc;
-    if ((rc = upload(c->status, gm->status, (size_t)n, st))) return rc;
-    if ((rc = upload(c->nbhs, gm->nbhs, (size_t)n, st))) return rc;
+    if ((rc = upload(c->status, status.data(), status.size(), st))) return rc;
+    if ((rc = upload(c->nbhs, nbhs.data(), nbhs.size(), st))) return rc;
     if ((rc = upload(c->conn, conn0.data(), conn0.size(), st))) return rc;
-    if ((rc = upload(c->split_cnt, cnt.data(), cnt.size(), st))) return rc;
-    if ((rc = upload(c->split, split0.data(), split0.size(), st))) return rc;
-    if ((rc = upload(c->alias, gm->alias, (size_t)n, st))) return rc;
+    if ((rc = upload(c->dir_cnt, cnt.data(), cnt.size(), st))) return rc;
+    if ((rc = upload(c->dir_off, off.data(), off.size(), st))) return rc;
+    if ((rc = upload(c->lst, lst.data(), lst.size(), st))) return rc;
+    if ((rc = upload(c->mask, mask.data(), mask.size(), st))) return rc;
+    if ((rc = upload(c->alias, alias.data(), alias.size(), st))) return rc;
     if ((rc = upload(c->respos, respos.data(), respos.size(), st))) return rc;
+    if ((rc = upload(c->perm, perm.data(), perm.size(), st))) return rc;
     if ((rc = upload(c->idx_interior, li.data(), li.size(), st))) return rc;
     if ((rc = upload(c->idx_wall, lw.data(), lw.size(), st))) return rc;
     if ((rc = upload(c->idx_outer, lo.data(), lo.size(), st))) return rc;
     if ((rc = upload(c->idx_sup_in, lsi.data(), lsi.size(), st))) return rc;
     if ((rc = upload(c->idx_sup_out, lso.data(), lso.size(), st))) return rc;
+    if ((rc = upload(c->idx_fused_interior, lfi.data(), lfi.size(), st))) return rc;
+    if ((rc = upload(c->idx_fused_wall, lfw.data(), lfw.size(), st))) return rc;
+    if ((rc = upload(c->idx_generic, lgen.data(), lgen.size(), st))) return rc;
     c->tl.n_interior = (int)li.size();
     c->tl.n_wall = (int)lw.size();
     c->tl.n_outer = (int)lo.size();
@@ -464,6 +606,16 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
     c->tl.outer = c->idx_outer.p;
     c->tl.sup_in = c->idx_sup_in.p;
     c->tl.sup_out = c->idx_sup_out.p;
+    c->tl.n_fused_interior = (int)lfi.size();
+    c->tl.n_fused_wall = (int)lfw.size();
+    c->tl.n_generic = (int)lgen.size();
+    c->tl.fused_interior = c->idx_fused_interior.p;
+    c->tl.fused_wall = c->idx_fused_wall.p;
+    c->tl.generic = c->idx_generic.p;
+
+    if (c->coef.alloc(18ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "stencil coefficients");
+    launch_stencil_coefficients(c->geom(), c->coef.p, c->fp.power, st);
+    if ((rc = launch_error("stencil coefficients"))) return rc;
 
     // flow state (zero until uploaded; initial_conditions.f90:23-24 zero delUp/delUn)
     DevBuf<double>* st5[] = {&c->prim, &c->U, &c->q, &c->flux_res, &c->delUp, &c->delUn};
@@ -472,21 +624,22 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
         HIP_TRY(hipMemsetAsync(b->p, 0, 5ULL * n * sizeof(double), st));
     }
     if (c->qm.alloc(10ULL * n) != hipSuccess || c->dq.alloc(15ULL * n) != hipSuccess ||
-        c->delt.alloc((size_t)n) != hipSuccess || c->dinv.alloc((size_t)n) != hipSuccess)
+        c->delt.alloc((size_t)n) != hipSuccess || c->dinv.alloc((size_t)n) != hipSuccess ||
+        c->staging.alloc(15ULL * n) != hipSuccess)
         return fail(MFCFD_ENOMEM, "state arrays");
     HIP_TRY(hipMemsetAsync(c->qm.p, 0, 10ULL * n * sizeof(double), st));
     HIP_TRY(hipMemsetAsync(c->dq.p, 0, 15ULL * n * sizeof(double), st));
     HIP_TRY(hipMemsetAsync(c->delt.p, 0, (size_t)n * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(c->dinv.p, 0, (size_t)n * sizeof(double), st));
     if (c->fp.inner_iterations > 0 && c->temp.alloc(15ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "temp");
     c->n_partial = state_update_blocks(n);
     if (c->partial.alloc(2ULL * c->n_partial) != hipSuccess || c->partial_pos.alloc(2ULL * c->n_partial) != hipSuccess)
         return fail(MFCFD_ENOMEM, "residual partials");
 
-    // sweep schedule over the alias order
+    // sweep schedule over the alias order (built in host numbering)
     std::vector<unsigned char> swept(n, 0);
     for (int i = 0; i < n; ++i) {
         const int p = gm->point_with_alias[i];
-        if (p < 0 || p > n) return fail(MFCFD_EINVAL, "point_with_alias(%d)=%d out of range", i + 1, p);
         if (p > 0 && (gm->status[p - 1] == 0 || gm->status[p - 1] == 1)) swept[p - 1] = 1;
     }
     c->implicit_supported = lsi.empty() && lso.empty();
@@ -503,8 +656,11 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
     in.pwa = gm->point_with_alias;
     in.swept = swept.data();
     c->sched.build(in);
-    if ((rc = upload(c->fwd_points, c->sched.fwd_points.data(), c->sched.fwd_points.size(), st))) return rc;
-    if ((rc = upload(c->bwd_points, c->sched.bwd_points.data(), c->sched.bwd_points.size(), st))) return rc;
+    std::vector<int> fwd(c->sched.fwd_points.size()), bwd(c->sched.bwd_points.size());
+    for (size_t e = 0; e < fwd.size(); ++e) fwd[e] = inv[c->sched.fwd_points[e]];
+    for (size_t e = 0; e < bwd.size(); ++e) bwd[e] = inv[c->sched.bwd_points[e]];
+    if ((rc = upload(c->fwd_points, fwd.data(), fwd.size(), st))) return rc;
+    if ((rc = upload(c->bwd_points, bwd.data(), bwd.size(), st))) return rc;
     HIP_TRY(hipStreamSynchronize(st));
     c->have_geom = true;
     c->res_old = c->res_new = c->residue = 0.0;
@@ -512,12 +668,46 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
     return MFCFD_OK;
 }
 
+int mfcfd_bind_geometry_arrays(mfcfd_ctx* c, int n, const double* x, const double* y, const double* z,
+                               const double* tan1, const double* tan2, const double* nor,
+                               const double* min_dist, const int* status, const int* nbhs, const int* conn,
+                               const int* xpos_nbhs, const int* xneg_nbhs, const int* ypos_nbhs,
+                               const int* yneg_nbhs, const int* zpos_nbhs, const int* zneg_nbhs,
+                               const int* xpos_conn, const int* xneg_conn, const int* ypos_conn,
+                               const int* yneg_conn, const int* zpos_conn, const int* zneg_conn,
+                               const int* alias, const int* point_with_alias) {
+    mfcfd_geometry g;
+    g.n = n;
+    g.x = x;
+    g.y = y;
+    g.z = z;
+    g.tan1 = tan1;
+    g.tan2 = tan2;
+    g.nor = nor;
+    g.min_dist = min_dist;
+    g.status = status;
+    g.nbhs = nbhs;
+    g.conn = conn;
+    const int* cnts[6] = {xpos_nbhs, xneg_nbhs, ypos_nbhs, yneg_nbhs, zpos_nbhs, zneg_nbhs};
+    const int* lists[6] = {xpos_conn, xneg_conn, ypos_conn, yneg_conn, zpos_conn, zneg_conn};
+    for (int d = 0; d < 6; ++d) {
+        g.split_nbhs[d] = cnts[d];
+        g.split_conn[d] = lists[d];
+    }
+    g.alias = alias;
+    g.point_with_alias = point_with_alias;
+    return mfcfd_bind_geometry(c, &g);
+}
+
 int mfcfd_upload(mfcfd_ctx* c, int id, const double* host) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
     ArrayDesc a = array_of(c, id);
     if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
-    HIP_TRY(hipMemcpyAsync(a.buf->p, host, a.per_point * c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
+    HIP_TRY(hipMemcpyAsync(c->staging.p, host, bytes, hipMemcpyHostToDevice, c->stream));
+    launch_gather(a.buf->p, c->staging.p, c->perm.p, c->n, a.per_point, c->stream);
+    if ((rc = launch_error("upload permutation"))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MFCFD_OK;
 }
@@ -527,7 +717,10 @@ int mfcfd_download(mfcfd_ctx* c, int id, double* host) {
     if (rc) return rc;
     ArrayDesc a = array_of(c, id);
     if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
-    HIP_TRY(hipMemcpyAsync(host, a.buf->p, a.per_point * c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
+    launch_scatter(c->staging.p, a.buf->p, c->perm.p, c->n, a.per_point, c->stream);
+    if ((rc = launch_error("download permutation"))) return rc;
+    HIP_TRY(hipMemcpyAsync(host, c->staging.p, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MFCFD_OK;
 }
@@ -563,7 +756,7 @@ int mfcfd_timestep_delt(mfcfd_ctx* c) {
 int mfcfd_eval_flux_residual(mfcfd_ctx* c) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
-    launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream);
+    flux(c, kFluxAll);
     return launch_error("eval_flux_residual");
 }
 
@@ -649,9 +842,14 @@ int mfcfd_time_phases(mfcfd_ctx* c, int t0, int n_iters, double* out_ms) {
             launch_qderiv_timestep(g, s, c->fp, c->stream);
         }
         hipEventRecord(ev[2], c->stream);
-        launch_flux_residual(g, s, c->fp, c->tl, c->stream, 1);
-        hipEventRecord(ev[3], c->stream);
-        launch_flux_residual(g, s, c->fp, c->tl, c->stream, 6);
+        if (c->params.flux_order == MFCFD_FLUX_LISTS) {
+            launch_flux_residual_reference(g, s, c->fp, c->tl, c->stream);
+            hipEventRecord(ev[3], c->stream);
+        } else {
+            flux(c, 1);
+            hipEventRecord(ev[3], c->stream);
+            flux(c, 6);
+        }
         hipEventRecord(ev[4], c->stream);
         if ((rc = launch_error("phases"))) break;
         if ((rc = do_sweeps(c))) break;
@@ -681,6 +879,14 @@ int mfcfd_sweep_info(mfcfd_ctx* c, int* levels_fwd, int* levels_bwd, int* n_swep
     if (levels_fwd) *levels_fwd = c->sched.n_levels_fwd;
     if (levels_bwd) *levels_bwd = c->sched.n_levels_bwd;
     if (n_swept) *n_swept = (int)c->sched.order.size();
+    return MFCFD_OK;
+}
+
+int mfcfd_point_classes(mfcfd_ctx* c, int* n_fused_interior, int* n_fused_wall, int* n_listwise) {
+    if (!c || !c->have_geom) return fail(MFCFD_ESTATE, "no geometry bound");
+    if (n_fused_interior) *n_fused_interior = c->tl.n_fused_interior;
+    if (n_fused_wall) *n_fused_wall = c->tl.n_fused_wall;
+    if (n_listwise) *n_listwise = c->tl.n_generic;
     return MFCFD_OK;
 }
 

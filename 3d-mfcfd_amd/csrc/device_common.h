@@ -1,0 +1,72 @@
+// device_common.h -- helpers shared by the FP64 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_state.h"
+#include "qlskum_math.h"
+
+namespace mfcfd {
+
+constexpr int kBlock = 256;
+
+inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
+
+__device__ __forceinline__ Frame<double> load_frame(const double* frame, int k) {
+    Frame<double> f;
+    const double* p = frame + 9 * (long long)k;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        f.t1[c] = p[c];
+        f.t2[c] = p[3 + c];
+        f.n[c] = p[6 + c];
+    }
+    return f;
+}
+
+// least-squares weight 1/dist**power; UNIT_W selects the power = 0 kernels,
+// where the weight is exactly 1 and no power function is compiled in
+template <bool UNIT_W>
+__device__ __forceinline__ double wls_weight(double a, double b, double c, double power) {
+    if (UNIT_W) return 1.0;
+    return 1.0 / pow(__builtin_sqrt(a * a + b * b + c * c), power);
+}
+
+// Limited reconstruction of q at one end of an edge and conversion to
+// primitive variables (interior_flux_dGxpos.f90:92-96,99-103):
+//   qtilde = q - 0.5*phi*(delx*dq1 + dely*dq2 + delz*dq3)
+__device__ __forceinline__ void reconstruct_end(const double q[5], const double* dq, const double* qm,
+                                                double epsi2, double delx, double dely, double delz,
+                                                double prim[5]) {
+    double temp[5], qt[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        temp[r] = delx * dq[3 * r] + dely * dq[3 * r + 1] + delz * dq[3 * r + 2];
+        qt[r] = q[r] - 0.5 * temp[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const double phi = venkat_phi(qt[r], q[r], qm[2 * r], qm[2 * r + 1], epsi2);
+        qt[r] = q[r] - 0.5 * phi * temp[r];
+    }
+    qtilde_to_primitive(qt, prim);
+}
+
+// epsi**2 of the limiter, epsi = (VL_CONST*min_dist)**3 (limiters.f90:34-35)
+__device__ __forceinline__ double limiter_epsi2(double vl_const, double min_dist) {
+    double e = vl_const * min_dist;
+    e = e * e * e;
+    return e * e;
+}
+
+// Flux family and final division of direction d for a point class
+// (flux_residual.f90:40-81): interior half-range fluxes; wall quarter-range
+// Gw for x/y and half-range Gzn over conn for z; outer quarter-range Go for
+// x/y and half-range Gzp, both with temp*(1/det).
+__device__ __forceinline__ int dir_family(int cls, int d) {
+    if (cls == 1) return d <= YNEG ? 1 : 0;
+    if (cls == 2) return d <= YNEG ? 2 : 0;
+    return 0;
+}
+
+}  // namespace mfcfd

@@ -1,16 +1,30 @@
 // device_state.h -- the HBM-resident point cloud and flow state.
 //
-// Layout (per point k, 0-based):
+// Points are stored in device order: a Morton (Z-order) permutation of the
+// host's point numbering, so the 25 neighbours of a point usually share
+// cache lines with the neighbours of the points next to it.  The host's
+// numbering survives only at the boundary (upload/download permute, the
+// residual reports host point numbers).
+//
+// Per point k (device order, 0-based):
 //   pos[k]            double4 (x, y, z, min_dist): one 32-byte gather per neighbour
 //   frame[9k..9k+8]   tan1(3), tan2(3), nor(3)
-//   conn[27k + j]     0-based neighbour numbers (data_structure.f90:111 order)
-//   split[d][22k + j] 0-based split lists, addressed exactly like the
-//                     reference's column-major (22,N) arrays so that lists
-//                     longer than 22 read the next point's column (the
-//                     buffers carry 32 entries of slack)
+//   conn[27k + j]     neighbours in the host's conn order (data_structure.f90:111)
+//   dir_cnt[d*n + k], dir_off[d*n + k]
+//                     direction d = xpos, xneg, ypos, yneg, zpos, zneg: the list
+//                     each reference stencil of point k walks, resolved once at
+//                     bind time from the host's (22,N) arrays exactly as the
+//                     reference reads them (a list longer than 22 continues into
+//                     the next point's column).  Wall points store their full
+//                     conn as zneg (wall_flux_dGzneg.f90:61-65), outer points
+//                     their zpos list; lst[dir_off .. dir_off + dir_cnt) holds it
+//   mask[27k + j]     for regular points (every list an in-order subsequence of
+//                     conn): bit d set when conn entry j belongs to list d
+//   coef[18k + 3d ..] Cramer weights of list d: the derivative of that stencil is
+//                     sum_j (G_k - G_i) * (c0*dels_j + c1*delt_j + c2*deln_j)
 //   prim/U/q/flux_res/delUp/delUn [5k + r], qm [(5k + r)*2 + s],
-//   dq [(5k + r)*3 + c], delt [k]  -- the reference's own (5,N), (2,5,N),
-//                     (3,5,N) layouts, so host transfers are plain copies.
+//   dq [(5k + r)*3 + c], delt [k]  -- the reference's (5,N), (2,5,N), (3,5,N)
+//                     layouts, so transfers are plain permuted copies.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,11 +44,14 @@ struct DevGeom {
     const int* status;
     const int* nbhs;
     const int* conn;
-    const int* split_cnt;   // [6][n]
-    const int* split;       // [6][split_stride]
-    long long split_stride;
+    const int* dir_cnt;          // [6][n]
+    const long long* dir_off;    // [6][n]
+    const int* lst;
+    const unsigned char* mask;   // [27n]
+    const double* coef;          // [18n]
     const int* alias;
-    const int* respos;      // position of a point in the residual order (wall, then interior)
+    const int* respos;           // position in the residual order (wall, then interior)
+    const int* perm;             // device point -> host point (0-based)
 };
 
 struct DevState {
@@ -61,7 +78,7 @@ struct ResidualRecord {
     double res_new;
     double sum_res_sqr;
     double max_res;
-    int max_res_point;   // 1-based, as the reference stores it
+    int max_res_point;   // host numbering, 1-based (0: no positive residual)
     int pad;
 };
 

@@ -7,13 +7,20 @@
 
 namespace mfcfd {
 
+// device point numbers of each class
 struct TypeLists {
     int n_interior, n_wall, n_outer, n_sup_in, n_sup_out;
-    const int* interior;  // 0-based point numbers
+    const int* interior;
     const int* wall;
     const int* outer;
     const int* sup_in;
     const int* sup_out;
+    // flux residual work split: regular interior / regular wall points take
+    // the fused stencil kernel, everything else the list-by-list kernel
+    int n_fused_interior, n_fused_wall, n_generic;
+    const int* fused_interior;
+    const int* fused_wall;
+    const int* generic;
 };
 
 struct SweepSchedule {
@@ -25,6 +32,12 @@ struct SweepSchedule {
     int n_swept;
 };
 
+// setup: Cramer weights of every direction list (geometry only)
+void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, hipStream_t st);
+// boundary permutations: dst[k*m + c] = src[perm[k]*m + c] and the inverse
+void launch_gather(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
+void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
+
 // compute_conserved_vector.f90 / q_variables.f90
 void launch_conserved_vector(const DevGeom& g, const DevState& s, hipStream_t st);
 void launch_q_variables(const DevGeom& g, const DevState& s, hipStream_t st);
@@ -34,10 +47,13 @@ void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams&
                           hipStream_t st);
 void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
 void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
-// flux_residual.f90; class_mask bit 0 interior, bit 1 wall, bit 2 outer
+// flux_residual.f90; part bit 0: fused interior, bit 1: fused wall, bit 2: list-by-list
 constexpr int kFluxAll = 7;
 void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p,
-                          const TypeLists& tl, hipStream_t st, int class_mask = kFluxAll);
+                          const TypeLists& tl, hipStream_t st, int parts = kFluxAll);
+// reference-order flux residual of every point of a class (parity path)
+void launch_flux_residual_reference(const DevGeom& g, const DevState& s, const FlowParams& p,
+                                    const TypeLists& tl, hipStream_t st);
 // explicit form of the sweep (delUn = -delt*flux_res, delUp = delUn)
 void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st);
 // implicit_forward_and_reverse_sweeps.f90

@@ -1,11 +1,15 @@
-// kernels_fp64.hip -- FP64 kernels of the q-LSKUM pseudo-time iteration.
+// kernels_fp64.hip -- FP64 kernels of the q-LSKUM pseudo-time iteration in
+// the reference's evaluation order.
 //
-// One thread per point; points of one class (interior, wall, outer) are
-// launched together so a wavefront never mixes stencil shapes.  Expression
-// order follows the reference routine named at each kernel; this file is
-// compiled with -ffp-contract=off.
+// One thread per point; points of one class are launched together.  Every
+// list-walking routine below follows the reference routine named at it with
+// the same association of every sum and product; the file is compiled with
+// -ffp-contract=off.  The fused flux kernels (kernels_flux_fused.hip) replace
+// the flux residual for regular points in the iteration; this file keeps the
+// list-by-list form for the remaining points and for the parity entry point.
 #include <hip/hip_runtime.h>
 
+#include "device_common.h"
 #include "device_state.h"
 #include "kernels.h"
 #include "qlskum_math.h"
@@ -14,21 +18,67 @@ namespace mfcfd {
 
 namespace {
 
-constexpr int kBlock = 256;
-
-__device__ __forceinline__ Frame<double> load_frame(const double* frame, int k) {
-    Frame<double> f;
-    const double* p = frame + 9 * (long long)k;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        f.t1[c] = p[c];
-        f.t2[c] = p[3 + c];
-        f.n[c] = p[6 + c];
-    }
-    return f;
+// ---------------------------------------------------------------------------
+// boundary permutations
+// ---------------------------------------------------------------------------
+__global__ void k_gather(double* dst, const double* src, const int* perm, int n, int m) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= (long long)n * m) return;
+    const long long k = e / m;
+    const int c = (int)(e - k * m);
+    dst[e] = src[(long long)perm[k] * m + c];
 }
 
-inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
+__global__ void k_scatter(double* dst, const double* src, const int* perm, int n, int m) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= (long long)n * m) return;
+    const long long k = e / m;
+    const int c = (int)(e - k * m);
+    dst[(long long)perm[k] * m + c] = src[e];
+}
+
+// ---------------------------------------------------------------------------
+// Cramer weights of every direction list: the geometric sums of the list in
+// list order (interior_flux_dGxpos.f90:84-90) and the cofactors of the column
+// the direction solves for, divided by the determinant
+// ---------------------------------------------------------------------------
+template <bool UW>
+__global__ void __launch_bounds__(kBlock) k_stencil_coef(DevGeom g, double* coef, double power) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const Frame<double> f = load_frame(g.frame, i);
+    const double4 pi4 = g.pos[i];
+    for (int d = 0; d < 6; ++d) {
+        const int cnt = g.dir_cnt[(long long)d * g.n + i];
+        const int* list = g.lst + g.dir_off[(long long)d * g.n + i];
+        Geo6<double> geo = {0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < cnt; ++j) {
+            const double4 pk = g.pos[list[j]];
+            const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
+            const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+            const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+            const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+            const double w = wls_weight<UW>(dels, delt, deln, power);
+            const double dels_w = dels * w, delt_w = delt * w, deln_w = deln * w;
+            geo.xx = geo.xx + dels * dels_w;
+            geo.yy = geo.yy + delt * delt_w;
+            geo.zz = geo.zz + deln * deln_w;
+            geo.xy = geo.xy + dels * delt_w;
+            geo.yz = geo.yz + delt * deln_w;
+            geo.zx = geo.zx + deln * dels_w;
+        }
+        const int col = d >> 1;
+        const double det = cramer_det(geo);
+        double* c = coef + 18 * (long long)i + 3 * d;
+        if (cnt == 0) {
+            c[0] = c[1] = c[2] = 0.0;
+            continue;
+        }
+        c[0] = cramer_num(geo, col, 1.0, 0.0, 0.0) / det;
+        c[1] = cramer_num(geo, col, 0.0, 1.0, 0.0) / det;
+        c[2] = cramer_num(geo, col, 0.0, 0.0, 1.0) / det;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // compute_conserved_vector.f90:20-32 and q_variables.f90:18-38
@@ -73,8 +123,11 @@ __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
 // ---------------------------------------------------------------------------
 // q_derivatives.f90:17-110 (first pass) and timestep_delt.f90:13-38
 // ---------------------------------------------------------------------------
-template <bool DO_DQ, bool DO_DT>
-__global__ void __launch_bounds__(kBlock) k_qderiv_timestep(DevGeom g, DevState s, FlowParams fp) {
+#ifndef MFCFD_QDERIV_WAVES
+#define MFCFD_QDERIV_WAVES 4
+#endif
+template <bool DO_DQ, bool DO_DT, bool UW>
+__global__ void __launch_bounds__(kBlock, MFCFD_QDERIV_WAVES) k_qderiv_timestep(DevGeom g, DevState s, FlowParams fp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
     const double4 pi4 = g.pos[i];
@@ -99,7 +152,6 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_timestep(DevGeom g, DevState 
         const double delx = pk.x - pi4.x;
         const double dely = pk.y - pi4.y;
         const double delz = pk.z - pi4.z;
-        const double dist = __builtin_sqrt(delx * delx + dely * dely + delz * delz);
         if (DO_DQ) {
             double qk[5];
 #pragma unroll
@@ -109,7 +161,7 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_timestep(DevGeom g, DevState 
                 if (qk[r] > qmax[r]) qmax[r] = qk[r];
                 if (qk[r] < qmin[r]) qmin[r] = qk[r];
             }
-            const double weights = 1.0 / pow(dist, fp.power);
+            const double weights = wls_weight<UW>(delx, dely, delz, fp.power);
             geo.xx = geo.xx + delx * delx * weights;
             geo.yy = geo.yy + dely * dely * weights;
             geo.zz = geo.zz + delz * delz * weights;
@@ -125,6 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_timestep(DevGeom g, DevState 
             }
         }
         if (DO_DT) {
+            const double dist = __builtin_sqrt(delx * delx + dely * dely + delz * delz);
             const double* pk5 = s.prim + 5 * (long long)k;
             const double rho = pk5[0], u1 = pk5[1], u2 = pk5[2], u3 = pk5[3], pr = pk5[4];
             const double mod_u = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3);
@@ -148,6 +201,7 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_timestep(DevGeom g, DevState 
 }
 
 // q_derivatives.f90:116-201: inner iterations (second-order q-derivatives)
+template <bool UW>
 __global__ void __launch_bounds__(kBlock) k_qderiv_inner(DevGeom g, DevState s, FlowParams fp,
                                                          double* temp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,8 +217,7 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_inner(DevGeom g, DevState s, 
         const double delx = pk.x - pi4.x;
         const double dely = pk.y - pi4.y;
         const double delz = pk.z - pi4.z;
-        const double dist = __builtin_sqrt(delx * delx + dely * dely + delz * delz);
-        const double weights = 1.0 / pow(dist, fp.power);
+        const double weights = wls_weight<UW>(delx, dely, delz, fp.power);
         geo.xx = geo.xx + delx * delx * weights;
         geo.yy = geo.yy + dely * dely * weights;
         geo.zz = geo.zz + delz * delz * weights;
@@ -198,11 +251,11 @@ __global__ void k_copy(double* dst, const double* src, long long n) {
 }
 
 // ---------------------------------------------------------------------------
-// flux stencils: interior_flux_dG*.f90, wall_flux_dG*.f90, outer_flux_dG*.f90
+// flux stencils in list order: interior_flux_dG*.f90, wall_flux_dG*.f90,
+// outer_flux_dG*.f90
 // ---------------------------------------------------------------------------
 struct Stencil {
     const int* list;
-    int stride;   // 22 for split lists, 27 for conn
     int count;
     int family;   // 0 half-range, 1 wall octant, 2 outer octant
     int axis;
@@ -210,40 +263,29 @@ struct Stencil {
     bool by_one_by_det;
 };
 
-// limited reconstruction at one end of edge (i,k) and its flux in the frame
-// of i (interior_flux_dGxpos.f90:92-104)
-__device__ __forceinline__ void edge_end_flux(const DevGeom& g, const DevState& s, const FlowParams& fp,
-                                              int at, double delx, double dely, double delz,
-                                              const Stencil& st, const Frame<double>& f, double G[5]) {
-    double temp[5], qt[5], prim[5];
-    const double* q = s.q + 5 * (long long)at;
-    const double* dq = s.dq + 15 * (long long)at;
-    const double* qm = s.qm + 10 * (long long)at;
-    double epsi = fp.vl_const * g.pos[at].w;
-    epsi = epsi * epsi * epsi;
-    const double epsi2 = epsi * epsi;
-#pragma unroll
-    for (int r = 0; r < 5; ++r) {
-        temp[r] = delx * dq[3 * r] + dely * dq[3 * r + 1] + delz * dq[3 * r + 2];
-        qt[r] = q[r] - 0.5 * temp[r];
-    }
-#pragma unroll
-    for (int r = 0; r < 5; ++r) {
-        const double phi = venkat_phi(qt[r], q[r], qm[2 * r], qm[2 * r + 1], epsi2);
-        qt[r] = q[r] - 0.5 * phi * temp[r];
-    }
-    qtilde_to_primitive(qt, prim);
-    flux_eval(st.family, st.axis, st.sign, f, prim, fp.pi, G);
+__device__ __forceinline__ Stencil dir_stencil(const DevGeom& g, int cls, int d, int i) {
+    Stencil st;
+    st.list = g.lst + g.dir_off[(long long)d * g.n + i];
+    st.count = g.dir_cnt[(long long)d * g.n + i];
+    st.family = dir_family(cls, d);
+    st.axis = d >> 1;
+    st.sign = (d & 1) ? -1 : 1;
+    st.by_one_by_det = cls == 2;
+    return st;
 }
 
+template <bool UW>
 __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
                            const Stencil& st, const Frame<double>& f, double G[5]) {
     Geo6<double> geo = {0, 0, 0, 0, 0, 0};
     double sx[5] = {0, 0, 0, 0, 0}, sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0};
     const double4 pi4 = g.pos[i];
-    const int* list = st.list + (long long)st.stride * i;
+    double qi[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) qi[r] = s.q[5 * (long long)i + r];
+    const double epsi2_i = limiter_epsi2(fp.vl_const, pi4.w);
     for (int j = 0; j < st.count; ++j) {
-        const int k = list[j];
+        const int k = st.list[j];
         const double4 pk = g.pos[k];
         const double delx = pk.x - pi4.x;
         const double dely = pk.y - pi4.y;
@@ -251,8 +293,7 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
         const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
         const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
         const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
-        const double dist = __builtin_sqrt(dels * dels + delt * delt + deln * deln);
-        const double weights = 1.0 / pow(dist, fp.power);
+        const double weights = wls_weight<UW>(dels, delt, deln, fp.power);
         const double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
         geo.xx = geo.xx + dels * dels_w;
         geo.yy = geo.yy + delt * delt_w;
@@ -260,9 +301,15 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
         geo.xy = geo.xy + dels * delt_w;
         geo.yz = geo.yz + delt * deln_w;
         geo.zx = geo.zx + deln * dels_w;
-        double Gi[5], Gk[5];
-        edge_end_flux(g, s, fp, i, delx, dely, delz, st, f, Gi);
-        edge_end_flux(g, s, fp, k, delx, dely, delz, st, f, Gk);
+        double prim[5], Gi[5], Gk[5], qk[5];
+        reconstruct_end(qi, s.dq + 15 * (long long)i, s.qm + 10 * (long long)i, epsi2_i, delx, dely, delz,
+                        prim);
+        flux_eval(st.family, st.axis, st.sign, f, prim, fp.pi, Gi);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) qk[r] = s.q[5 * (long long)k + r];
+        reconstruct_end(qk, s.dq + 15 * (long long)k, s.qm + 10 * (long long)k,
+                        limiter_epsi2(fp.vl_const, pk.w), delx, dely, delz, prim);
+        flux_eval(st.family, st.axis, st.sign, f, prim, fp.pi, Gk);
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
             const double t = Gk[r] - Gi[r];
@@ -282,68 +329,30 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
     }
 }
 
-__device__ __forceinline__ Stencil split_stencil(const DevGeom& g, int dir, int i, int family) {
-    Stencil st;
-    st.list = g.split + g.split_stride * dir;
-    st.stride = kSplitCols;
-    st.count = g.split_cnt[(long long)dir * g.n + i];
-    st.family = family;
-    st.axis = dir >> 1;
-    st.sign = (dir & 1) ? -1 : 1;
-    st.by_one_by_det = family == 2;
-    return st;
-}
-
-__device__ __forceinline__ Stencil conn_stencil(const DevGeom& g, int i, int axis, int sign) {
-    Stencil st;
-    st.list = g.conn;
-    st.stride = kConnCols;
-    st.count = g.nbhs[i];
-    st.family = 0;
-    st.axis = axis;
-    st.sign = sign;
-    st.by_one_by_det = false;
-    return st;
-}
-
-// flux_residual.f90:40-81; CLASS 0 interior, 1 wall, 2 outer
-template <int CLASS>
-__global__ void __launch_bounds__(kBlock) k_flux_residual(DevGeom g, DevState s, FlowParams fp,
-                                                          const int* idx, int count) {
+// flux_residual.f90:40-81 for the points of idx (any class, from status)
+template <bool UW>
+__global__ void __launch_bounds__(kBlock) k_flux_residual_lists(DevGeom g, DevState s, FlowParams fp,
+                                                                const int* idx, int count) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= count) return;
     const int i = idx[m];
+    const int cls = g.status[i];
     const Frame<double> f = load_frame(g.frame, i);
     double acc[5], G[5];
-    if (CLASS == 0) {
-        stencil_dG(g, s, fp, i, split_stencil(g, XPOS, i, 0), f, acc);
-        for (int d = XNEG; d <= ZNEG; ++d) {
-            stencil_dG(g, s, fp, i, split_stencil(g, d, i, 0), f, G);
+    stencil_dG<UW>(g, s, fp, i, dir_stencil(g, cls, XPOS, i), f, acc);
+    const int last = cls == 0 ? ZNEG : YNEG;
+#pragma unroll 1
+    for (int d = XNEG; d <= last; ++d) {
+        stencil_dG<UW>(g, s, fp, i, dir_stencil(g, cls, d, i), f, G);
 #pragma unroll
-            for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
-        }
-    } else if (CLASS == 1) {
-        stencil_dG(g, s, fp, i, split_stencil(g, XPOS, i, 1), f, acc);
-        for (int d = XNEG; d <= YNEG; ++d) {
-            stencil_dG(g, s, fp, i, split_stencil(g, d, i, 1), f, G);
-#pragma unroll
-            for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
-        }
-        // wall_flux_dGzneg.f90:61-65: full conn, half-range flux_Gzn
-        stencil_dG(g, s, fp, i, conn_stencil(g, i, 2, -1), f, G);
+        for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
+    }
+    if (cls == 1) {
+        stencil_dG<UW>(g, s, fp, i, dir_stencil(g, 1, ZNEG, i), f, G);
 #pragma unroll
         for (int r = 0; r < 5; ++r) acc[r] = 2.0 * (acc[r] + G[r]);
-    } else {
-        stencil_dG(g, s, fp, i, split_stencil(g, XPOS, i, 2), f, acc);
-        for (int d = XNEG; d <= YNEG; ++d) {
-            stencil_dG(g, s, fp, i, split_stencil(g, d, i, 2), f, G);
-#pragma unroll
-            for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
-        }
-        // outer_flux_dGzpos.f90: zpos list, flux_Gzp, temp*(1/det)
-        Stencil st = split_stencil(g, ZPOS, i, 0);
-        st.by_one_by_det = true;
-        stencil_dG(g, s, fp, i, st, f, G);
+    } else if (cls == 2) {
+        stencil_dG<UW>(g, s, fp, i, dir_stencil(g, 2, ZPOS, i), f, G);
 #pragma unroll
         for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
     }
@@ -368,23 +377,21 @@ __global__ void __launch_bounds__(kBlock) k_explicit_sweep(DevGeom g, DevState s
 // implicit LU-SGS: implicit_D_inv_{interior,wall}.f90 and the alias-ordered
 // sweeps of implicit_forward_and_reverse_sweeps.f90:26-88
 // ---------------------------------------------------------------------------
+template <bool UW>
 __device__ double dinv_dir(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
                            const Stencil& st, const Frame<double>& f, double as) {
     Geo6<double> geo = {0, 0, 0, 0, 0, 0};
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const double4 pi4 = g.pos[i];
-    const int* list = st.list + (long long)st.stride * i;
     for (int j = 0; j < st.count; ++j) {
-        const int k = list[j];
-        const double4 pk = g.pos[k];
+        const double4 pk = g.pos[st.list[j]];
         const double delx = pk.x - pi4.x;
         const double dely = pk.y - pi4.y;
         const double delz = pk.z - pi4.z;
         const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
         const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
         const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
-        const double dist = __builtin_sqrt(dels * dels + delt * delt + deln * deln);
-        const double weights = 1.0 / pow(dist, fp.power);
+        const double weights = wls_weight<UW>(dels, delt, deln, fp.power);
         const double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
         geo.xx = geo.xx + dels * dels_w;
         geo.yy = geo.yy + delt * delt_w;
@@ -409,23 +416,24 @@ __device__ double dinv_dir(const DevGeom& g, const DevState& s, const FlowParams
     return -(0.50 * fabs(temp1 - fabs(temp1)) * temp / det);
 }
 
+template <bool UW>
 __global__ void __launch_bounds__(kBlock) k_dinv(DevGeom g, DevState s, FlowParams fp, const int* idx,
                                                  int count) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= count) return;
     const int i = idx[m];
-    const int st_i = g.status[i];
+    const int cls = g.status[i];
     const Frame<double> f = load_frame(g.frame, i);
     const double delta_t = s.delt[i];
     const double as = __builtin_sqrt(1.40 * s.prim[5 * (long long)i + 4] / s.prim[5 * (long long)i]);
     double d[6];
     double temp;
-    if (st_i == 1) {
-        for (int dd = XPOS; dd <= YNEG; ++dd) d[dd] = dinv_dir(g, s, fp, i, split_stencil(g, dd, i, 1), f, as);
-        d[4] = dinv_dir(g, s, fp, i, conn_stencil(g, i, 2, -1), f, as);
+    if (cls == 1) {
+        for (int dd = XPOS; dd <= YNEG; ++dd) d[dd] = dinv_dir<UW>(g, s, fp, i, dir_stencil(g, 1, dd, i), f, as);
+        d[4] = dinv_dir<UW>(g, s, fp, i, dir_stencil(g, 1, ZNEG, i), f, as);
         temp = (1.0 / delta_t) - 2.0 * (d[0] + d[1] + d[2] + d[3] + d[4]);
     } else {
-        for (int dd = XPOS; dd <= ZNEG; ++dd) d[dd] = dinv_dir(g, s, fp, i, split_stencil(g, dd, i, 0), f, as);
+        for (int dd = XPOS; dd <= ZNEG; ++dd) d[dd] = dinv_dir<UW>(g, s, fp, i, dir_stencil(g, 0, dd, i), f, as);
         temp = (1.0 / delta_t) - (d[0] + d[1] + d[2] + d[3] + d[4] + d[5]);
     }
     s.dinv[i] = 1.0 / temp;
@@ -433,17 +441,16 @@ __global__ void __launch_bounds__(kBlock) k_dinv(DevGeom g, DevState s, FlowPara
 
 // one direction of LdelU (FORWARD) or UdelU
 // (implicit_LdelU_forward_sweep_interior.f90:51-117)
-template <bool FORWARD>
+template <bool FORWARD, bool UW>
 __device__ void sweep_dir(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
                           const Stencil& st, const Frame<double>& f, double dG[5]) {
     Geo6<double> geo = {0, 0, 0, 0, 0, 0};
     double sx[5] = {0, 0, 0, 0, 0}, sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0};
     const double4 pi4 = g.pos[i];
     const int alias_i = g.alias[i];
-    const int* list = st.list + (long long)st.stride * i;
     const double* dU = FORWARD ? s.delUp : s.delUn;
     for (int j = 0; j < st.count; ++j) {
-        const int k = list[j];
+        const int k = st.list[j];
         const double4 pk = g.pos[k];
         const double delx = pk.x - pi4.x;
         const double dely = pk.y - pi4.y;
@@ -451,8 +458,7 @@ __device__ void sweep_dir(const DevGeom& g, const DevState& s, const FlowParams&
         const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
         const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
         const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
-        const double dist = __builtin_sqrt(dels * dels + delt * delt + deln * deln);
-        const double weights = 1.0 / pow(dist, fp.power);
+        const double weights = wls_weight<UW>(dels, delt, deln, fp.power);
         const double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
         geo.xx = geo.xx + dels * dels_w;
         geo.yy = geo.yy + delt * delt_w;
@@ -486,31 +492,33 @@ __device__ void sweep_dir(const DevGeom& g, const DevState& s, const FlowParams&
     for (int r = 0; r < 5; ++r) dG[r] = cramer_num(geo, st.axis, sx[r], sy[r], sz[r]) / det;
 }
 
-template <bool FORWARD>
+template <bool FORWARD, bool UW>
 __device__ void sweep_term(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
                            const Frame<double>& f, double out[5]) {
     double d[5];
     if (g.status[i] == 1) {
-        sweep_dir<FORWARD>(g, s, fp, i, split_stencil(g, XPOS, i, 1), f, out);
+        sweep_dir<FORWARD, UW>(g, s, fp, i, dir_stencil(g, 1, XPOS, i), f, out);
+#pragma unroll 1
         for (int dd = XNEG; dd <= YNEG; ++dd) {
-            sweep_dir<FORWARD>(g, s, fp, i, split_stencil(g, dd, i, 1), f, d);
+            sweep_dir<FORWARD, UW>(g, s, fp, i, dir_stencil(g, 1, dd, i), f, d);
 #pragma unroll
             for (int r = 0; r < 5; ++r) out[r] = out[r] + d[r];
         }
-        sweep_dir<FORWARD>(g, s, fp, i, conn_stencil(g, i, 2, -1), f, d);
+        sweep_dir<FORWARD, UW>(g, s, fp, i, dir_stencil(g, 1, ZNEG, i), f, d);
 #pragma unroll
         for (int r = 0; r < 5; ++r) out[r] = 2.0 * (out[r] + d[r]);
         return;
     }
-    sweep_dir<FORWARD>(g, s, fp, i, split_stencil(g, XPOS, i, 0), f, out);
+    sweep_dir<FORWARD, UW>(g, s, fp, i, dir_stencil(g, 0, XPOS, i), f, out);
+#pragma unroll 1
     for (int dd = XNEG; dd <= ZNEG; ++dd) {
-        sweep_dir<FORWARD>(g, s, fp, i, split_stencil(g, dd, i, 0), f, d);
+        sweep_dir<FORWARD, UW>(g, s, fp, i, dir_stencil(g, 0, dd, i), f, d);
 #pragma unroll
         for (int r = 0; r < 5; ++r) out[r] = out[r] + d[r];
     }
 }
 
-template <bool FORWARD>
+template <bool FORWARD, bool UW>
 __global__ void __launch_bounds__(64) k_sweep_level(DevGeom g, DevState s, FlowParams fp,
                                                     const int* points, int begin, int end) {
     const int m = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -518,7 +526,7 @@ __global__ void __launch_bounds__(64) k_sweep_level(DevGeom g, DevState s, FlowP
     const int p = points[m];
     const Frame<double> f = load_frame(g.frame, p);
     double T5[5];
-    sweep_term<FORWARD>(g, s, fp, p, f, T5);
+    sweep_term<FORWARD, UW>(g, s, fp, p, f, T5);
     const double D_inv = s.dinv[p];
     if (FORWARD) {
 #pragma unroll
@@ -534,7 +542,7 @@ __global__ void __launch_bounds__(64) k_sweep_level(DevGeom g, DevState s, FlowP
 // ---------------------------------------------------------------------------
 // state_update.f90:22-115 with the residual of state_update.f90:39-44,115
 // ---------------------------------------------------------------------------
-__device__ void conserved_vector_Ubar(const DevGeom& g, const DevState& s, const FlowParams& fp, int k,
+__device__ void conserved_vector_Ubar(const DevState& s, const FlowParams& fp, int k,
                                       const Frame<double>& f, double Ubar[5]) {
     // conserved_vector_Ubar.f90:19-58
     const double pi = fp.pi;
@@ -602,7 +610,7 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
             res_sqr = (U[0] - temp) * (U[0] - temp);
             mp.v = res_sqr;
             mp.pos = g.respos[k];
-            mp.point = k + 1;
+            mp.point = g.perm[k] + 1;
             double prim[5];
             conserved_to_prim(U, f, prim);
 #pragma unroll
@@ -613,7 +621,7 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
         } else if (st == 2) {
             const Frame<double> f = load_frame(g.frame, k);
             double U[5], prim[5];
-            conserved_vector_Ubar(g, s, fp, k, f, U);
+            conserved_vector_Ubar(s, fp, k, f, U);
             const double dt = s.delt[k];
 #pragma unroll
             for (int r = 0; r < 5; ++r) U[r] = U[r] - dt * s.flux_res[5 * (long long)k + r];
@@ -686,7 +694,7 @@ __global__ void __launch_bounds__(1024) k_residual_final(const double* partial, 
     __shared__ int spt[1024];
     double acc = 0.0;
     MaxPair mp = {0.0, 0x7fffffff, 0};
-    // fixed assignment of partials to threads and fixed tree: deterministic
+    // fixed assignment of partials to threads and a fixed tree: deterministic
     for (int b = threadIdx.x; b < n_partial; b += blockDim.x) {
         acc = acc + partial[b];
         MaxPair o = {partial_max[b], partial_pos[b], partial_pt[b]};
@@ -723,9 +731,31 @@ __global__ void __launch_bounds__(1024) k_residual_final(const double* partial, 
 
 }  // namespace
 
+// implemented in kernels_flux_fused.hip
+void launch_flux_fused(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
+                       hipStream_t st, int parts);
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
+void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, hipStream_t st) {
+    if (g.n <= 0) return;
+    if (power == 0.0)
+        hipLaunchKernelGGL(k_stencil_coef<true>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, coef, power);
+    else
+        hipLaunchKernelGGL(k_stencil_coef<false>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, coef,
+                           power);
+}
+void launch_gather(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather, dim3(grid_for((long long)n * m, kBlock)), dim3(kBlock), 0, st, dst, src, perm,
+                       n, m);
+}
+void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for((long long)n * m, kBlock)), dim3(kBlock), 0, st, dst, src,
+                       perm, n, m);
+}
 void launch_conserved_vector(const DevGeom& g, const DevState& s, hipStream_t st) {
     if (g.n <= 0) return;
     hipLaunchKernelGGL((k_conserved_q<true, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
@@ -741,35 +771,63 @@ void launch_conserved_and_q(const DevGeom& g, const DevState& s, hipStream_t st)
 void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams& p, double* temp,
                           hipStream_t st) {
     if (g.n <= 0) return;
-    hipLaunchKernelGGL((k_qderiv_timestep<true, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
-                       g, s, p);
+    if (p.power == 0.0)
+        hipLaunchKernelGGL((k_qderiv_timestep<true, false, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
+                           st, g, s, p);
+    else
+        hipLaunchKernelGGL((k_qderiv_timestep<true, false, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
+                           st, g, s, p);
     for (int it = 0; it < p.inner_iterations; ++it) {
-        hipLaunchKernelGGL(k_qderiv_inner, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s, p, temp);
+        if (p.power == 0.0)
+            hipLaunchKernelGGL(k_qderiv_inner<true>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s, p,
+                               temp);
+        else
+            hipLaunchKernelGGL(k_qderiv_inner<false>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s, p,
+                               temp);
         hipLaunchKernelGGL(k_copy, dim3(grid_for(15LL * g.n, kBlock)), dim3(kBlock), 0, st, s.dq,
                            (const double*)temp, 15LL * g.n);
     }
 }
 void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
     if (g.n <= 0) return;
-    hipLaunchKernelGGL((k_qderiv_timestep<false, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
+    // no least-squares weights in the time step
+    hipLaunchKernelGGL((k_qderiv_timestep<false, true, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
                        g, s, p);
 }
 void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
     if (g.n <= 0) return;
-    hipLaunchKernelGGL((k_qderiv_timestep<true, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
-                       g, s, p);
+    if (p.power == 0.0)
+        hipLaunchKernelGGL((k_qderiv_timestep<true, true, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
+                           st, g, s, p);
+    else
+        hipLaunchKernelGGL((k_qderiv_timestep<true, true, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
+                           st, g, s, p);
 }
 void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
-                          hipStream_t st, int class_mask) {
-    if ((class_mask & 2) && tl.n_wall > 0)
-        hipLaunchKernelGGL(k_flux_residual<1>, dim3(grid_for(tl.n_wall, kBlock)), dim3(kBlock), 0, st, g, s,
-                           p, tl.wall, tl.n_wall);
-    if ((class_mask & 4) && tl.n_outer > 0)
-        hipLaunchKernelGGL(k_flux_residual<2>, dim3(grid_for(tl.n_outer, kBlock)), dim3(kBlock), 0, st, g,
-                           s, p, tl.outer, tl.n_outer);
-    if ((class_mask & 1) && tl.n_interior > 0)
-        hipLaunchKernelGGL(k_flux_residual<0>, dim3(grid_for(tl.n_interior, kBlock)), dim3(kBlock), 0, st,
-                           g, s, p, tl.interior, tl.n_interior);
+                          hipStream_t st, int parts) {
+    launch_flux_fused(g, s, p, tl, st, parts);
+    if ((parts & 4) && tl.n_generic > 0) {
+        if (p.power == 0.0)
+            hipLaunchKernelGGL(k_flux_residual_lists<true>, dim3(grid_for(tl.n_generic, kBlock)), dim3(kBlock), 0,
+                               st, g, s, p, tl.generic, tl.n_generic);
+        else
+            hipLaunchKernelGGL(k_flux_residual_lists<false>, dim3(grid_for(tl.n_generic, kBlock)), dim3(kBlock), 0,
+                               st, g, s, p, tl.generic, tl.n_generic);
+    }
+}
+void launch_flux_residual_reference(const DevGeom& g, const DevState& s, const FlowParams& p,
+                                    const TypeLists& tl, hipStream_t st) {
+    const int* lists[3] = {tl.wall, tl.outer, tl.interior};
+    const int counts[3] = {tl.n_wall, tl.n_outer, tl.n_interior};
+    for (int c = 0; c < 3; ++c)
+        if (counts[c] > 0) {
+            if (p.power == 0.0)
+                hipLaunchKernelGGL(k_flux_residual_lists<true>, dim3(grid_for(counts[c], kBlock)), dim3(kBlock), 0,
+                                   st, g, s, p, lists[c], counts[c]);
+            else
+                hipLaunchKernelGGL(k_flux_residual_lists<false>, dim3(grid_for(counts[c], kBlock)), dim3(kBlock),
+                                   0, st, g, s, p, lists[c], counts[c]);
+        }
 }
 void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st) {
     if (g.n <= 0) return;
@@ -777,24 +835,37 @@ void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st) 
 }
 void launch_dinv(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                  hipStream_t st) {
-    if (tl.n_wall > 0)
-        hipLaunchKernelGGL(k_dinv, dim3(grid_for(tl.n_wall, kBlock)), dim3(kBlock), 0, st, g, s, p, tl.wall,
-                           tl.n_wall);
-    if (tl.n_interior > 0)
-        hipLaunchKernelGGL(k_dinv, dim3(grid_for(tl.n_interior, kBlock)), dim3(kBlock), 0, st, g, s, p,
-                           tl.interior, tl.n_interior);
+    const int* lists[2] = {tl.wall, tl.interior};
+    const int counts[2] = {tl.n_wall, tl.n_interior};
+    for (int c = 0; c < 2; ++c) {
+        if (counts[c] <= 0) continue;
+        if (p.power == 0.0)
+            hipLaunchKernelGGL(k_dinv<true>, dim3(grid_for(counts[c], kBlock)), dim3(kBlock), 0, st, g, s, p,
+                               lists[c], counts[c]);
+        else
+            hipLaunchKernelGGL(k_dinv<false>, dim3(grid_for(counts[c], kBlock)), dim3(kBlock), 0, st, g, s, p,
+                               lists[c], counts[c]);
+    }
 }
 void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepSchedule& sch,
                           hipStream_t st) {
     for (int l = 0; l < sch.n_levels_fwd; ++l) {
         const int b = sch.fwd_ptr[l], e = sch.fwd_ptr[l + 1];
-        hipLaunchKernelGGL(k_sweep_level<true>, dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
-                           sch.fwd_points, b, e);
+        if (p.power == 0.0)
+            hipLaunchKernelGGL((k_sweep_level<true, true>), dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
+                               sch.fwd_points, b, e);
+        else
+            hipLaunchKernelGGL((k_sweep_level<true, false>), dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
+                               sch.fwd_points, b, e);
     }
     for (int l = 0; l < sch.n_levels_bwd; ++l) {
         const int b = sch.bwd_ptr[l], e = sch.bwd_ptr[l + 1];
-        hipLaunchKernelGGL(k_sweep_level<false>, dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
-                           sch.bwd_points, b, e);
+        if (p.power == 0.0)
+            hipLaunchKernelGGL((k_sweep_level<false, true>), dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
+                               sch.bwd_points, b, e);
+        else
+            hipLaunchKernelGGL((k_sweep_level<false, false>), dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
+                               sch.bwd_points, b, e);
     }
     // implicit_forward_and_reverse_sweeps.f90:86-88
     hipLaunchKernelGGL(k_copy, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,

@@ -42,7 +42,7 @@ class Params(C.Structure):
     _fields_ = [("mach", C.c_double), ("aoa", C.c_double), ("cfl", C.c_double),
                 ("vl_const", C.c_double), ("power", C.c_double),
                 ("inner_iterations", C.c_int), ("mode", C.c_int), ("precision", C.c_int),
-                ("max_iters", C.c_int)]
+                ("max_iters", C.c_int), ("flux_order", C.c_int)]
 
 
 class Geometry(C.Structure):
@@ -67,7 +67,7 @@ EXPORTS = ["mfcfd_create", "mfcfd_destroy", "mfcfd_last_error", "mfcfd_abi_versi
            "mfcfd_eval_q_derivatives", "mfcfd_timestep_delt", "mfcfd_eval_flux_residual",
            "mfcfd_implicit_forward_backward_sweeps", "mfcfd_state_update", "mfcfd_fpi_solver",
            "mfcfd_run", "mfcfd_get_residual", "mfcfd_synchronize", "mfcfd_sweep_info",
-           "mfcfd_time_phases"]
+           "mfcfd_time_phases", "mfcfd_point_classes"]
 PHASES = ["conserved_q", "qderiv_timestep", "flux_interior", "flux_boundary", "sweeps",
           "state_update"]
 
@@ -100,6 +100,7 @@ def lib():
         L.mfcfd_get_residual.argtypes = [vp, C.POINTER(Residual)]
         L.mfcfd_sweep_info.argtypes = [vp, _ip, _ip, _ip]
         L.mfcfd_time_phases.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.mfcfd_point_classes.argtypes = [vp, _ip, _ip, _ip]
         _lib = L
     return _lib
 
@@ -163,7 +164,7 @@ class Solver:
 
     def __init__(self, cloud, mach, aoa, cfl, mode="explicit", precision="fp64",
                  vl_const=2.0, power=0.0, inner_iterations=0, max_iters=1000, device=0,
-                 split=None, alias=None):
+                 split=None, alias=None, flux_order="fused"):
         L = lib()
         self.n = cloud.n
         self.cloud = cloud
@@ -171,7 +172,8 @@ class Solver:
         _check(L.mfcfd_create(device, C.byref(self._ctx)))
         self.params = Params(mach, aoa, cfl, vl_const, power, inner_iterations,
                              IMPLICIT if mode == "implicit" else EXPLICIT,
-                             MIXED if precision == "mixed" else FP64, max_iters)
+                             MIXED if precision == "mixed" else FP64, max_iters,
+                             1 if flux_order == "lists" else 0)
         _check(L.mfcfd_set_params(self._ctx, C.byref(self.params)))
         self._arrs = _cloud_arrays(cloud)
         cnt, lists = split if split is not None else split_stencils(cloud)
@@ -263,6 +265,11 @@ class Solver:
         out = np.zeros(len(PHASES))
         _check(lib().mfcfd_time_phases(self._ctx, t0, n_iters, _d(out)))
         return dict(zip(PHASES, out.tolist()))
+
+    def point_classes(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().mfcfd_point_classes(self._ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return {"fused_interior": a.value, "fused_wall": b.value, "listwise": c.value}
 
     def sweep_info(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MFCFD_ABI_VERSION 1
+#define MFCFD_ABI_VERSION 2
 
 /* error codes */
 #define MFCFD_OK 0
@@ -65,7 +65,17 @@ typedef struct mfcfd_params {
     int mode;                /* MFCFD_EXPLICIT or MFCFD_IMPLICIT */
     int precision;           /* MFCFD_FP64 or MFCFD_MIXED */
     int max_iters;           /* max_iters (used by the Fortran drop-in's final download) */
+    int flux_order;          /* MFCFD_FLUX_FUSED (default) or MFCFD_FLUX_LISTS */
 } mfcfd_params;
+
+/* flux residual evaluation order:
+ *  FUSED  regular points walk conn once, one limited reconstruction per edge
+ *         end, each list's least-squares solve applied edge by edge through
+ *         its Cramer weights (same fluxes, different rounding);
+ *  LISTS  every point walks each direction list and solves the accumulated
+ *         3x3 system at the end, exactly as interior_flux_dGxpos.f90:61-123 */
+#define MFCFD_FLUX_FUSED 0
+#define MFCFD_FLUX_LISTS 1
 
 /* the point/connectivity arrays of data_structure.f90:11-28,49-56 as the
  * host holds them (all point numbers 1-based) */
@@ -107,6 +117,19 @@ int mfcfd_set_params(mfcfd_ctx *ctx, const mfcfd_params *p);
  * generate_split_stencils (generate_connectivity.f90:12-34) and aliasing
  * (implicit_aliasing.f90:13-44).  Also builds the sweep schedule. */
 int mfcfd_bind_geometry(mfcfd_ctx *ctx, const mfcfd_geometry *g);
+
+/* the same binding with one argument per array, for hosts that pass arrays
+ * by reference (a Fortran bind(C) interface passes point%x, point%tan1, ...
+ * directly); split lists in xpos, xneg, ypos, yneg, zpos, zneg order */
+int mfcfd_bind_geometry_arrays(mfcfd_ctx *ctx, int n, const double *x, const double *y,
+                               const double *z, const double *tan1, const double *tan2,
+                               const double *nor, const double *min_dist, const int *status,
+                               const int *nbhs, const int *conn, const int *xpos_nbhs,
+                               const int *xneg_nbhs, const int *ypos_nbhs, const int *yneg_nbhs,
+                               const int *zpos_nbhs, const int *zneg_nbhs, const int *xpos_conn,
+                               const int *xneg_conn, const int *ypos_conn, const int *yneg_conn,
+                               const int *zpos_conn, const int *zneg_conn, const int *alias,
+                               const int *point_with_alias);
 
 /* host preprocessing drop-ins: generate_split_stencils
  * (generate_connectivity.f90:12-246) and aliasing (implicit_aliasing.f90:
@@ -159,6 +182,11 @@ int mfcfd_time_phases(mfcfd_ctx *ctx, int t0, int n_iters, double *out_ms);
 /* diagnostics: sweep schedule depth (levels of the forward and backward
  * Gauss-Seidel dependency chains) and swept point count */
 int mfcfd_sweep_info(mfcfd_ctx *ctx, int *levels_fwd, int *levels_bwd, int *n_swept);
+
+/* diagnostics: how many points take the fused flux kernel (interior, wall)
+ * and how many the list-by-list kernel (outer points and points whose lists
+ * were altered by the 22-column spill) */
+int mfcfd_point_classes(mfcfd_ctx *ctx, int *n_fused_interior, int *n_fused_wall, int *n_listwise);
 
 #ifdef __cplusplus
 }

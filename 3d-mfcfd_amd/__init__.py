@@ -14,5 +14,6 @@ import sys as _sys
 
 from . import pointcloud  # noqa: F401
 from . import solver  # noqa: F401
+from . import postprocess  # noqa: F401
 
 _sys.modules.setdefault("mfcfd_amd", _sys.modules[__name__])

@@ -29,6 +29,7 @@ UNITS = {
     "capi.hip": ["-ffp-contract=off"],
     "kernels_fp64.hip": ["-ffp-contract=off"],
     "kernels_flux_fused.hip": ["-ffp-contract=off"],
+    "kernels_sweep.hip": ["-ffp-contract=off"],
     "preprocess.cpp": ["-ffp-contract=off", "-x", "hip"],
 }
 

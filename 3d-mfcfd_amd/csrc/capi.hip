@@ -137,6 +137,11 @@ struct mfcfd_ctx {
     Schedule sched;
     std::vector<int> fwd_ptr, bwd_ptr;
     DevBuf<int> fwd_points, bwd_points;
+    // sync-free sweeps: points in sweep order, dependency lists, flags
+    DevBuf<int> fwd_seq, bwd_seq, fwd_dep, bwd_dep, sweep_flag;
+    DevBuf<long long> fwd_dep_ptr, bwd_dep_ptr;
+    DevBuf<unsigned> sweep_ctr;
+    int sweep_grid = 0;
     bool implicit_supported = true;
     // bookkeeping of fpi_solver.f90:38-43
     double res_old = 0.0, res_new = 0.0, residue = 0.0, max_res = 0.0, sum_res_sqr = 0.0;
@@ -181,8 +186,12 @@ struct mfcfd_ctx {
         for (auto* b : dbl) b->release();
         DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
-                               &idx_fused_wall, &idx_generic, &partial_pos, &fwd_points, &bwd_points};
+                               &idx_fused_wall, &idx_generic, &partial_pos, &fwd_points, &bwd_points,
+                               &fwd_seq, &bwd_seq, &fwd_dep, &bwd_dep, &sweep_flag};
         for (auto* b : ints) b->release();
+        fwd_dep_ptr.release();
+        bwd_dep_ptr.release();
+        sweep_ctr.release();
         dir_off.release();
         mask.release();
         pos.release();
@@ -263,7 +272,22 @@ int do_sweeps(mfcfd_ctx* c) {
     sch.fwd_points = c->fwd_points.p;
     sch.bwd_points = c->bwd_points.p;
     sch.n_swept = (int)c->sched.order.size();
-    launch_sweeps_levels(g, s, c->fp, sch, c->stream);
+    if (c->params.sweep_schedule == MFCFD_SWEEP_LEVELS) {
+        launch_sweeps_levels(g, s, c->fp, sch, c->stream);
+        return launch_error("implicit sweeps");
+    }
+    SweepAsync fwd, bwd;
+    fwd.count = bwd.count = sch.n_swept;
+    fwd.flag = bwd.flag = c->sweep_flag.p;
+    fwd.ctr = bwd.ctr = c->sweep_ctr.p;
+    fwd.stalls = bwd.stalls = &c->rec.p->stalls;
+    fwd.seq = c->fwd_seq.p;
+    fwd.dep_ptr = c->fwd_dep_ptr.p;
+    fwd.dep = c->fwd_dep.p;
+    bwd.seq = c->bwd_seq.p;
+    bwd.dep_ptr = c->bwd_dep_ptr.p;
+    bwd.dep = c->bwd_dep.p;
+    launch_sweeps_async(g, s, c->fp, fwd, bwd, c->sweep_grid, c->stream);
     return launch_error("implicit sweeps");
 }
 
@@ -309,6 +333,8 @@ int fetch_record(mfcfd_ctx* c) {
     c->sum_res_sqr = c->host_rec->sum_res_sqr;
     c->max_res = c->host_rec->max_res;
     if (c->host_rec->max_res_point > 0) c->max_res_point = c->host_rec->max_res_point;
+    if (c->host_rec->stalls)
+        return fail(MFCFD_EHIP, "implicit sweep: %d dependency waits exceeded the spin limit", c->host_rec->stalls);
     return MFCFD_OK;
 }
 
@@ -376,6 +402,8 @@ int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
     if (p->precision == MFCFD_MIXED) return fail(MFCFD_EUNSUPPORTED, "mixed precision is not built in");
     if (p->flux_order != MFCFD_FLUX_FUSED && p->flux_order != MFCFD_FLUX_LISTS)
         return fail(MFCFD_EINVAL, "flux_order %d", p->flux_order);
+    if (p->sweep_schedule != MFCFD_SWEEP_ASYNC && p->sweep_schedule != MFCFD_SWEEP_LEVELS)
+        return fail(MFCFD_EINVAL, "sweep_schedule %d", p->sweep_schedule);
     if (!(p->cfl > 0.0)) return fail(MFCFD_EINVAL, "CFL must be positive");
     if (p->inner_iterations < 0) return fail(MFCFD_EINVAL, "inner_iterations < 0");
     if (c->have_geom && p->power != c->params.power)
@@ -661,6 +689,44 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
     for (size_t e = 0; e < bwd.size(); ++e) bwd[e] = inv[c->sched.bwd_points[e]];
     if ((rc = upload(c->fwd_points, fwd.data(), fwd.size(), st))) return rc;
     if ((rc = upload(c->bwd_points, bwd.data(), bwd.size(), st))) return rc;
+    {
+        const Schedule& S = c->sched;
+        const size_t m = S.order.size();
+        std::vector<int> seq_f(m), seq_b(m), dep_f(S.fwd_dep.size()), dep_b(S.bwd_dep.size());
+        for (size_t t = 0; t < m; ++t) {
+            seq_f[t] = inv[S.order[t]];
+            seq_b[t] = inv[S.order[m - 1 - t]];
+        }
+        for (size_t e = 0; e < dep_f.size(); ++e) dep_f[e] = inv[S.fwd_dep[e]];
+        for (size_t e = 0; e < dep_b.size(); ++e) dep_b[e] = inv[S.bwd_dep[e]];
+        // every dependency of a sweep position must sit at an earlier
+        // position: the sync-free sweeps wait on nothing else
+        std::vector<int> at(n, -1);
+        for (size_t t = 0; t < m; ++t) at[S.order[t]] = (int)t;
+        for (size_t t = 0; t < m; ++t) {
+            for (long long e = S.fwd_dep_ptr[t]; e < S.fwd_dep_ptr[t + 1]; ++e) {
+                const int q = at[S.fwd_dep[e]];
+                if (q < 0 || q >= (int)t) return fail(MFCFD_EINVAL, "forward sweep dependency out of order");
+            }
+            const size_t tb = m - 1 - t;   // backward position t is order[m-1-t]
+            for (long long e = S.bwd_dep_ptr[t]; e < S.bwd_dep_ptr[t + 1]; ++e) {
+                const int q = at[S.bwd_dep[e]];
+                if (q < 0 || q <= (int)tb) return fail(MFCFD_EINVAL, "backward sweep dependency out of order");
+            }
+        }
+        if ((rc = upload(c->fwd_seq, seq_f.data(), m, st))) return rc;
+        if ((rc = upload(c->bwd_seq, seq_b.data(), m, st))) return rc;
+        if ((rc = upload(c->fwd_dep, dep_f.data(), dep_f.size(), st))) return rc;
+        if ((rc = upload(c->bwd_dep, dep_b.data(), dep_b.size(), st))) return rc;
+        if ((rc = upload(c->fwd_dep_ptr, S.fwd_dep_ptr.data(), S.fwd_dep_ptr.size(), st))) return rc;
+        if ((rc = upload(c->bwd_dep_ptr, S.bwd_dep_ptr.data(), S.bwd_dep_ptr.size(), st))) return rc;
+        if (c->sweep_flag.alloc((size_t)n) != hipSuccess || c->sweep_ctr.alloc(2) != hipSuccess)
+            return fail(MFCFD_ENOMEM, "sweep flags");
+        HIP_TRY(hipMemsetAsync(c->sweep_flag.p, 0, (size_t)n * sizeof(int), st));
+        HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, 2 * sizeof(unsigned), st));
+        HIP_TRY(hipMemsetAsync(c->rec.p, 0, sizeof(ResidualRecord), st));
+        c->sweep_grid = sweep_async_grid((int)m, c->params.power == 0.0);
+    }
     HIP_TRY(hipStreamSynchronize(st));
     c->have_geom = true;
     c->res_old = c->res_new = c->residue = 0.0;

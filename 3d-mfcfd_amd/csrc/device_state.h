@@ -79,7 +79,7 @@ struct ResidualRecord {
     double sum_res_sqr;
     double max_res;
     int max_res_point;   // host numbering, 1-based (0: no positive residual)
-    int pad;
+    int stalls;          // sync-free sweep waits that hit their spin limit
 };
 
 }  // namespace mfcfd

@@ -32,6 +32,17 @@ struct SweepSchedule {
     int n_swept;
 };
 
+// sync-free sweeps (kernels_sweep.hip); one per sweep direction
+struct SweepAsync {
+    const int* seq;             // swept points (device numbers) in sweep order
+    const long long* dep_ptr;   // [count+1]: dependencies of position t are dep[dep_ptr[t] .. dep_ptr[t+1])
+    const int* dep;             // device point numbers
+    int count;
+    int* flag;                  // [n] per-point completion counter
+    unsigned* ctr;              // [2] ticket counter, finished workgroups (zero between sweeps)
+    int* stalls;                // waits that hit the spin limit
+};
+
 // setup: Cramer weights of every direction list (geometry only)
 void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, hipStream_t st);
 // boundary permutations: dst[k*m + c] = src[perm[k]*m + c] and the inverse
@@ -61,6 +72,11 @@ void launch_dinv(const DevGeom& g, const DevState& s, const FlowParams& p, const
                  hipStream_t st);
 void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams& p,
                           const SweepSchedule& sch, hipStream_t st);
+// forward and backward sweeps without level barriers, then delUp = delUn
+void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
+                         const SweepAsync& bwd, int grid, hipStream_t st);
+// resident workgroup count used for the sync-free sweeps
+int sweep_async_grid(int count, bool unit_weight);
 // state_update.f90 (+ residual reduction into *rec; hist may be null)
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p,
                          const TypeLists& tl, double* partial, int* partial_pos, int n_partial,

@@ -1,0 +1,248 @@
+// kernels_sweep.hip -- LU-SGS sweeps without level barriers.
+//
+// The forward sweep of implicit_forward_and_reverse_sweeps.f90:26-57 visits
+// the swept points in alias order; point i reads delUp of every stencil
+// neighbour with a smaller alias, already updated earlier in the same sweep
+// (implicit_LdelU_forward_sweep_interior.f90:51-117).  The backward sweep
+// (implicit_forward_and_reverse_sweeps.f90:58-85) mirrors it with delUn and
+// larger aliases.  The result is a Gauss-Seidel recurrence whose dependency
+// graph is fixed by the alias order.
+//
+// Here a persistent grid of workgroups takes tickets from a counter; ticket
+// t processes position t of the sweep order.  Every point a workgroup waits
+// on has a smaller position, so it belongs to a workgroup that took its
+// ticket earlier and is running: the lowest unfinished ticket never waits
+// and every wait ends.  A point publishes its delU by bumping its flag with
+// release semantics at agent scope; a workgroup first computes everything
+// that does not depend on this sweep's values (geometry, Cramer weights,
+// the old fluxes Gold), then spins with acquire loads on the flags of its
+// dependencies.  Every swept point is bumped exactly once per sweep, so when
+// a sweep starts all swept points carry equal flags and "finished in this
+// sweep" is flag[k] != flag[i].  A wait that exceeds the spin limit is
+// counted in the residual record and the host reports an error; the kernel
+// still runs to completion.
+//
+// One workgroup of 128 lanes handles one point: lane e takes entry e of the
+// point's direction lists flattened in direction order (6 lists for an
+// interior point, x/y lists and the z list over conn for a wall point), and
+// the sum over all lists (the sweep term is the sum of the directions'
+// dG, times 2 at wall points) is a fixed-order tree reduction.  Each entry's
+// contribution is (Gnew - Gold) * w (c0*dels + c1*delt + c2*deln) with the
+// Cramer cofactors c of the list (k_stencil_coef), which equals the
+// reference's Cramer solve on the summed products up to rounding.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "device_state.h"
+#include "kernels.h"
+#include "qlskum_math.h"
+
+namespace mfcfd {
+
+namespace {
+
+constexpr int kSweepBlock = 128;
+constexpr int kSweepWaves = kSweepBlock / 64;
+// about a second of polling; legitimate waits end within milliseconds
+constexpr int kSpinLimit = 1 << 20;
+
+__device__ __forceinline__ int flag_acquire(const int* f) {
+    return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int flag_relaxed(const int* f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// wait until point k has finished in this sweep; gives up after the spin
+// limit, or at once when another wait of this sweep already gave up, so a
+// broken schedule ends the kernel instead of stalling it point by point
+__device__ __forceinline__ void wait_done(const int* fl, int my_flag, int* stalls) {
+    int spins = 0;
+    while (flag_acquire(fl) == my_flag) {
+        ++spins;
+        if ((spins & 255) == 0 && __hip_atomic_load(stalls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return;
+        if (spins > kSpinLimit) {
+            __hip_atomic_fetch_add(stalls, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <bool FORWARD, bool UW>
+__global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState s, FlowParams fp,
+                                                             SweepAsync a) {
+    __shared__ unsigned sh_ticket;
+    __shared__ double sh_part[kSweepWaves][5];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const double* dU = FORWARD ? s.delUp : s.delUn;
+    for (;;) {
+        if (tid == 0)
+            sh_ticket = __hip_atomic_fetch_add(a.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const unsigned t = sh_ticket;
+        if (t >= (unsigned)a.count) break;
+        const int i = a.seq[t];
+        const int cls = g.status[i];
+        const int alias_i = g.alias[i];
+        const Frame<double> f = load_frame(g.frame, i);
+        const double4 pi4 = g.pos[i];
+        int cnt[6];
+        long long off[6];
+        int total = 0;
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            cnt[d] = g.dir_cnt[(long long)d * g.n + i];
+            off[d] = g.dir_off[(long long)d * g.n + i];
+        }
+        if (cls == 1) cnt[ZPOS] = 0;  // a wall point's z term walks conn (the ZNEG list)
+#pragma unroll
+        for (int d = 0; d < 6; ++d) total += cnt[d];
+        const int my_flag = flag_relaxed(a.flag + i);
+        const long long db = a.dep_ptr[t], de = a.dep_ptr[t + 1];
+
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int base = 0; base < total; base += kSweepBlock) {
+            const int e = base + tid;
+            bool take = false;
+            int k = 0, d = 0;
+            double wgt = 0.0, Gold[5], Uk[5];
+            Frame<double> fk;
+            if (e < total) {
+                int rem = e;
+                while (rem >= cnt[d]) {
+                    rem -= cnt[d];
+                    ++d;
+                }
+                k = g.lst[off[d] + rem];
+                const int alias_k = g.alias[k];
+                take = FORWARD ? (alias_k < alias_i) : (alias_k > alias_i);
+                if (take) {
+                    const double4 pk = g.pos[k];
+                    const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
+                    const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+                    const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+                    const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+                    const double* c = g.coef + 18 * (long long)i + 3 * d;
+                    wgt = wls_weight<UW>(dels, delt, deln, fp.power) * (c[0] * dels + c[1] * delt + c[2] * deln);
+                    double prim[5];
+#pragma unroll
+                    for (int r = 0; r < 5; ++r) prim[r] = s.prim[5 * (long long)k + r];
+                    flux_eval(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, f, prim, fp.pi, Gold);
+#pragma unroll
+                    for (int r = 0; r < 5; ++r) Uk[r] = s.U[5 * (long long)k + r];
+                    fk = load_frame(g.frame, k);
+                }
+            }
+            if (base == 0) {
+                // wait for every dependency of this sweep position
+                for (long long q = db + tid; q < de; q += kSweepBlock) wait_done(a.flag + a.dep[q], my_flag, a.stalls);
+                __syncthreads();
+            }
+            if (take) {
+                double U[5], prim[5], Gnew[5];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) U[r] = dU[5 * (long long)k + r] + Uk[r];
+                conserved_to_prim(U, fk, prim);
+                flux_eval(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, f, prim, fp.pi, Gnew);
+#pragma unroll
+                for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (Gnew[r] - Gold[r]) * wgt;
+            }
+        }
+        if (total == 0) {
+            // no entries: still honour the dependency wait order
+            for (long long q = db + tid; q < de; q += kSweepBlock) wait_done(a.flag + a.dep[q], my_flag, a.stalls);
+        }
+#pragma unroll
+        for (int r = 0; r < 5; ++r) acc[r] = wave_sum(acc[r]);
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < 5; ++r) sh_part[wave][r] = acc[r];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double T5[5];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                double v = sh_part[0][r];
+#pragma unroll
+                for (int w = 1; w < kSweepWaves; ++w) v = v + sh_part[w][r];
+                T5[r] = cls == 1 ? 2.0 * v : v;
+            }
+            const double D_inv = s.dinv[i];
+            if (FORWARD) {
+#pragma unroll
+                for (int r = 0; r < 5; ++r)
+                    s.delUp[5 * (long long)i + r] = -(D_inv * (s.flux_res[5 * (long long)i + r] + T5[r]));
+            } else {
+#pragma unroll
+                for (int r = 0; r < 5; ++r)
+                    s.delUn[5 * (long long)i + r] = s.delUp[5 * (long long)i + r] - D_inv * T5[r];
+            }
+            __hip_atomic_store(a.flag + i, my_flag + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+    // the last workgroup out resets the counters for the next sweep
+    if (tid == 0) {
+        const unsigned done = __hip_atomic_fetch_add(a.ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (done == gridDim.x - 1) {
+            __hip_atomic_store(a.ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(a.ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+__global__ void k_copy_sweep(double* dst, const double* src, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+template <bool FORWARD, bool UW>
+int resident_grid(int count) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sweep_async<FORWARD, UW>,
+                                                     kSweepBlock, 0) != hipSuccess)
+        return 0;
+    if (per_cu < 1) per_cu = 1;
+    const long long grid = (long long)per_cu * prop.multiProcessorCount;
+    return (int)(grid < count ? grid : count);
+}
+
+}  // namespace
+
+int sweep_async_grid(int count, bool unit_weight) {
+    if (count <= 0) return 0;
+    const int a = unit_weight ? resident_grid<true, true>(count) : resident_grid<true, false>(count);
+    const int b = unit_weight ? resident_grid<false, true>(count) : resident_grid<false, false>(count);
+    return a < b ? a : b;
+}
+
+void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
+                         const SweepAsync& bwd, int grid, hipStream_t st) {
+    if (fwd.count > 0 && grid > 0) {
+        if (p.power == 0.0) {
+            hipLaunchKernelGGL((k_sweep_async<true, true>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, fwd);
+            hipLaunchKernelGGL((k_sweep_async<false, true>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, bwd);
+        } else {
+            hipLaunchKernelGGL((k_sweep_async<true, false>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, fwd);
+            hipLaunchKernelGGL((k_sweep_async<false, false>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, bwd);
+        }
+    }
+    // implicit_forward_and_reverse_sweeps.f90:86-88
+    hipLaunchKernelGGL(k_copy_sweep, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,
+                       (const double*)s.delUn, 5LL * g.n);
+}
+
+}  // namespace mfcfd

@@ -125,14 +125,19 @@ void Schedule::build(const Inputs& in) {
         if (in.status[p] == 0 || in.status[p] == 1) order.push_back(p);
     }
     std::vector<int> lev(n, 0), mark(n, -1), deps;
-    auto levels_for = [&](bool forward, std::vector<int>& ptr, std::vector<int>& pts, int& nlev) {
+    auto levels_for = [&](bool forward, std::vector<int>& ptr, std::vector<int>& pts, int& nlev,
+                          std::vector<long long>& dptr, std::vector<int>& dep) {
         std::fill(lev.begin(), lev.end(), 0);
         std::fill(mark.begin(), mark.end(), -1);
         int maxl = 0;
         const int m = (int)order.size();
+        dptr.assign(m + 1, 0);
+        dep.clear();
         for (int t = 0; t < m; ++t) {
             const int p = forward ? order[t] : order[m - 1 - t];
             for_each_dep(in, p, forward, deps, mark, p);
+            dep.insert(dep.end(), deps.begin(), deps.end());
+            dptr[t + 1] = (long long)dep.size();
             int l = 0;
             for (int k : deps) l = std::max(l, lev[k]);
             lev[p] = l + 1;
@@ -152,8 +157,8 @@ void Schedule::build(const Inputs& in) {
             pts[fill[lev[p] - 1]++] = p;
         }
     };
-    levels_for(true, fwd_ptr, fwd_points, n_levels_fwd);
-    levels_for(false, bwd_ptr, bwd_points, n_levels_bwd);
+    levels_for(true, fwd_ptr, fwd_points, n_levels_fwd, fwd_dep_ptr, fwd_dep);
+    levels_for(false, bwd_ptr, bwd_points, n_levels_bwd, bwd_dep_ptr, bwd_dep);
 }
 
 }  // namespace mfcfd

@@ -34,6 +34,10 @@ struct Schedule {
     std::vector<int> fwd_ptr, fwd_points;
     std::vector<int> bwd_ptr, bwd_points;
     int n_levels_fwd = 0, n_levels_bwd = 0;
+    // dependency lists by sweep position: forward position t is order[t],
+    // backward position t is order[M-1-t]; deps hold 0-based point numbers
+    std::vector<long long> fwd_dep_ptr, bwd_dep_ptr;
+    std::vector<int> fwd_dep, bwd_dep;
     void build(const Inputs& in);
 };
 

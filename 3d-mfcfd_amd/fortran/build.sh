@@ -3,12 +3,13 @@
 #
 # Compiles the reference's parameter_mod, data_structure_mod, host-side
 # routines and main program (meshfree_solver.f90, q_lskum.f90,
-# point_preprocessor.f90, initial_conditions.f90, post_processor.f90,
-# generate_connectivity.f90, implicit_aliasing.f90; read in place from the
-# reference directory) together with mfcfd_hip_binding.f90 and
-# mfcfd_dropin.f90, which replace every hot-path module, and links
-# libmfcfd_hip.so.  The program is the reference's unchanged driver running
-# the HIP hot path.  Because it contains compiled reference code, the
+# point_preprocessor.f90, initial_conditions.f90, generate_connectivity.f90,
+# implicit_aliasing.f90; read in place from the reference directory)
+# together with mfcfd_hip_binding.f90 and mfcfd_dropin.f90, which replace
+# every hot-path module, and mfcfd_postprocess.f90, which replaces
+# post_processor_mod (the reference's version stops on a Fortran unit
+# conflict under flang), and links libmfcfd_hip.so.  The program is the
+# reference's unchanged driver running the HIP hot path.  Because it contains compiled reference code, the
 # output goes to oracle/_ref/dropin_<tag>/ with the other reference builds.
 #
 # usage: build.sh [<tag> <max_points> <max_iters> <mach> <aoa> <cfl>]
@@ -43,7 +44,7 @@ one() {
         "$out/parameter.f90" "$REF/data_structure.f90"
         "$HERE/mfcfd_hip_binding.f90" "$HERE/mfcfd_dropin.f90"
         "$REF/generate_connectivity.f90" "$REF/implicit_aliasing.f90"
-        "$REF/point_preprocessor.f90" "$REF/initial_conditions.f90" "$REF/post_processor.f90"
+        "$REF/point_preprocessor.f90" "$REF/initial_conditions.f90" "$HERE/mfcfd_postprocess.f90"
         "$REF/q_lskum.f90" "$REF/meshfree_solver.f90"
     )
     local objs=()

@@ -13,13 +13,13 @@ module mfcfd_hip_binding
     implicit none
 
     integer(c_int), parameter :: MFCFD_EXPLICIT = 0, MFCFD_IMPLICIT = 1
-    integer(c_int), parameter :: MFCFD_FP64 = 0, MFCFD_FLUX_FUSED = 0
+    integer(c_int), parameter :: MFCFD_FP64 = 0, MFCFD_FLUX_FUSED = 0, MFCFD_SWEEP_ASYNC = 0
     integer(c_int), parameter :: ARR_PRIM = 0, ARR_U = 1, ARR_Q = 2, ARR_QM = 3, ARR_DQ = 4, &
                                  ARR_DELT = 5, ARR_FLUX_RES = 6, ARR_DELUP = 7, ARR_DELUN = 8
 
     type, bind(C) :: mfcfd_params_t
         real(c_double) :: mach, aoa, cfl, vl_const, power
-        integer(c_int) :: inner_iterations, mode, precision, max_iters, flux_order
+        integer(c_int) :: inner_iterations, mode, precision, max_iters, flux_order, sweep_schedule
     end type
 
     type, bind(C) :: mfcfd_residual_t
@@ -170,6 +170,7 @@ contains
         p%precision = MFCFD_FP64
         p%max_iters = max_iters
         p%flux_order = MFCFD_FLUX_FUSED
+        p%sweep_schedule = MFCFD_SWEEP_ASYNC
         call mfcfd_check(mfcfd_set_params(mfcfd_ctx, p), 'mfcfd_set_params')
         call mfcfd_check(mfcfd_bind_geometry_arrays(mfcfd_ctx, int(max_points, c_int), point%x, &
             point%y, point%z, point%tan1, point%tan2, point%nor, point%min_dist, point%status, &

@@ -639,14 +639,33 @@ def write_solution(path: str, cloud: PointCloud, prim: np.ndarray) -> None:
     np.savetxt(path, data, fmt="%.17g")
 
 
-def read_solution(path: str, n: int | None = None) -> np.ndarray:
-    """Read the prim columns of stored-solution.dat / primal-solution.dat -> (N, 5)."""
-    data = np.loadtxt(path, ndmin=2)
-    if data.shape[1] != 8:
-        raise ValueError(f"{path}: expected 8 columns, found {data.shape[1]}")
+def read_list_directed(path: str, per_record: int, n: int | None = None) -> np.ndarray:
+    """Values of a file of list-directed records, ``per_record`` values each.
+
+    Like a Fortran list-directed READ, values are taken in order whatever
+    the line breaks (a Fortran list-directed WRITE may wrap one record over
+    several lines); separators are blanks or commas and ``r*c`` stands for
+    r copies of c."""
+    vals = []
+    with open(path) as f:
+        for tok in f.read().replace(",", " ").split():
+            if "*" in tok:
+                r, c = tok.split("*", 1)
+                vals.extend([float(c)] * int(r))
+            else:
+                vals.append(float(tok))
+    if len(vals) % per_record:
+        raise ValueError(f"{path}: {len(vals)} values do not form records of {per_record}")
+    data = np.array(vals, dtype=np.float64).reshape(-1, per_record)
     if n is not None and data.shape[0] != n:
-        raise ValueError(f"{path}: expected {n} rows, found {data.shape[0]}")
-    return np.ascontiguousarray(data[:, 3:8])
+        raise ValueError(f"{path}: expected {n} records, found {data.shape[0]}")
+    return data
+
+
+def read_solution(path: str, n: int | None = None) -> np.ndarray:
+    """Read the prim columns of stored-solution.dat / primal-solution.dat -> (N, 5)
+    (records x, y, z, prim(1:5): initial_conditions.f90:26, post_processor.f90:34)."""
+    return np.ascontiguousarray(read_list_directed(path, 8, n)[:, 3:8])
 
 
 def save_npz(path: str, cloud: PointCloud) -> None:

@@ -42,7 +42,7 @@ class Params(C.Structure):
     _fields_ = [("mach", C.c_double), ("aoa", C.c_double), ("cfl", C.c_double),
                 ("vl_const", C.c_double), ("power", C.c_double),
                 ("inner_iterations", C.c_int), ("mode", C.c_int), ("precision", C.c_int),
-                ("max_iters", C.c_int), ("flux_order", C.c_int)]
+                ("max_iters", C.c_int), ("flux_order", C.c_int), ("sweep_schedule", C.c_int)]
 
 
 class Geometry(C.Structure):
@@ -164,7 +164,7 @@ class Solver:
 
     def __init__(self, cloud, mach, aoa, cfl, mode="explicit", precision="fp64",
                  vl_const=2.0, power=0.0, inner_iterations=0, max_iters=1000, device=0,
-                 split=None, alias=None, flux_order="fused"):
+                 split=None, alias=None, flux_order="fused", sweep_schedule="async"):
         L = lib()
         self.n = cloud.n
         self.cloud = cloud
@@ -173,7 +173,8 @@ class Solver:
         self.params = Params(mach, aoa, cfl, vl_const, power, inner_iterations,
                              IMPLICIT if mode == "implicit" else EXPLICIT,
                              MIXED if precision == "mixed" else FP64, max_iters,
-                             1 if flux_order == "lists" else 0)
+                             1 if flux_order == "lists" else 0,
+                             1 if sweep_schedule == "levels" else 0)
         _check(L.mfcfd_set_params(self._ctx, C.byref(self.params)))
         self._arrs = _cloud_arrays(cloud)
         cnt, lists = split if split is not None else split_stencils(cloud)

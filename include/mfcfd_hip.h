@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MFCFD_ABI_VERSION 2
+#define MFCFD_ABI_VERSION 3
 
 /* error codes */
 #define MFCFD_OK 0
@@ -66,6 +66,7 @@ typedef struct mfcfd_params {
     int precision;           /* MFCFD_FP64 or MFCFD_MIXED */
     int max_iters;           /* max_iters (used by the Fortran drop-in's final download) */
     int flux_order;          /* MFCFD_FLUX_FUSED (default) or MFCFD_FLUX_LISTS */
+    int sweep_schedule;      /* MFCFD_SWEEP_ASYNC (default) or MFCFD_SWEEP_LEVELS */
 } mfcfd_params;
 
 /* flux residual evaluation order:
@@ -76,6 +77,15 @@ typedef struct mfcfd_params {
  *         3x3 system at the end, exactly as interior_flux_dGxpos.f90:61-123 */
 #define MFCFD_FLUX_FUSED 0
 #define MFCFD_FLUX_LISTS 1
+
+/* schedule of the implicit LU-SGS sweeps; both visit each point's
+ * dependencies (stencil neighbours earlier in the sweep's alias order,
+ * implicit_forward_and_reverse_sweeps.f90:26-85) before the point itself:
+ *  ASYNC   one persistent launch per sweep; workgroups take points in sweep
+ *          order and wait on per-point completion flags;
+ *  LEVELS  one launch per dependency level (host-built level sets). */
+#define MFCFD_SWEEP_ASYNC 0
+#define MFCFD_SWEEP_LEVELS 1
 
 /* the point/connectivity arrays of data_structure.f90:11-28,49-56 as the
  * host holds them (all point numbers 1-based) */

@@ -38,8 +38,9 @@ def test_reference_driver_on_gpu_dropin(name):
         r = subprocess.run([exe], cwd=wd, env=env, capture_output=True, text=True, timeout=120)
         sys.stdout.write(r.stdout[-400:])
         assert r.returncode == 0, r.stderr[-2000:]
-        res = np.loadtxt(os.path.join(wd, "residue.dat"), ndmin=2)
+        res = pc.read_list_directed(os.path.join(wd, "residue.dat"), 3)
         prim = pc.read_solution(os.path.join(wd, "primal-solution.dat"), cl.n)
+        fres = pc.read_list_directed(os.path.join(wd, "flux-residual.dat"), 7, cl.n)
     assert res.shape[0] == iters
     ref = d["hist"][:, 0]
     nz = ref != 0.0
@@ -48,3 +49,7 @@ def test_reference_driver_on_gpu_dropin(name):
     assert np.max(np.abs(res[nz, 1] - ref[nz]) / np.abs(ref[nz])) < 1e-10
     scale = np.max(np.abs(d["final_prim"]), axis=0)
     assert np.max(np.abs(prim - d["final_prim"]) / scale) < 1e-9
+    assert np.array_equal(fres[:, 0].astype(int), np.arange(1, cl.n + 1))
+    assert np.array_equal(fres[:, 1].astype(int), cl.status)
+    fscale = np.max(np.abs(d["final_flux_res"]), axis=0)
+    assert np.max(np.abs(fres[:, 2:] - d["final_flux_res"]) / fscale) < 1e-8

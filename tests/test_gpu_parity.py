@@ -52,13 +52,16 @@ def make_solver(name, **kw):
 
 
 ORDERS = ["fused", "lists"]
+# the reference-order flux kernel runs with the level-set sweep schedule,
+# the fused kernel with the sync-free one
+SWEEP_OF = {"fused": "async", "lists": "levels"}
 
 
 @pytest.mark.parametrize("order", ORDERS)
 @pytest.mark.parametrize("name", FIXTURE_NAMES)
 def test_entry_points_match_reference(name, order):
     note(f"[gpu] entry points, fixture {name}, flux order {order}")
-    cl, d, s = make_solver(name, flux_order=order)
+    cl, d, s = make_solver(name, flux_order=order, sweep_schedule=SWEEP_OF[order])
     s.upload("prim", d["start_prim"])
     s.upload("delUp", d["start_delUp"])
     s.upload("delUn", d["start_delUn"])
@@ -95,7 +98,7 @@ def test_entry_points_match_reference(name, order):
 @pytest.mark.parametrize("name", FIXTURE_NAMES)
 def test_history_and_final_state_match_reference(name, order):
     note(f"[gpu] residual history, fixture {name}, flux order {order}")
-    cl, d, s = make_solver(name, flux_order=order)
+    cl, d, s = make_solver(name, flux_order=order, sweep_schedule=SWEEP_OF[order])
     iters = int(d["iterations"])
     rn, rs = s.run(1, iters)
     ref = d["hist"][:, 0]
@@ -162,6 +165,36 @@ def test_fused_and_list_orders_agree_on_stretched_cloud():
     nz = b != 0
     assert np.max(np.abs(a[nz] - b[nz]) / b[nz]) < 1e-12
     assert per_component_err(out["fused"][1], out["lists"][1]) < 1e-12
+
+
+def test_sweep_schedules_on_stretched_cloud():
+    """Implicit LU-SGS on a 20k-point stretched-shell cloud: the sync-free
+    sweeps against the C oracle (8 iterations, same bounds as the fixtures)
+    and against the level-set schedule (same dependencies, each list's solve
+    applied edge by edge instead of on the summed products: 1e-12)."""
+    note("[gpu] implicit sweeps, 20k shell cloud: async vs levels vs oracle")
+    cl = pc.ellipsoid_cloud(1600, 800, 17600, seed=22, order="morton")
+    mach, aoa, cfl, iters = 0.84, 3.06, 2.0, 8
+    prim0 = np.tile(pc.freestream_prim(mach, aoa), (cl.n, 1))
+    o = orc.Oracle(cl, mach, aoa, cfl, prim=prim0)
+    rn_o, _ = o.run(1, iters, True)
+    out = {}
+    for sweep in ("async", "levels"):
+        s = solver.Solver(cl, mach, aoa, cfl, mode="implicit", sweep_schedule=sweep)
+        s.upload("prim", prim0)
+        rn, _ = s.run(1, iters)
+        out[sweep] = (rn, s.download("prim"))
+        note(f"      {sweep}: sweep levels {s.sweep_info()}")
+        s.close()
+    nz = rn_o != 0
+    for sweep in out:
+        rn = out[sweep][0]
+        assert np.all(rn[~nz] == 0.0)
+        assert np.max(np.abs(rn[nz] - rn_o[nz]) / rn_o[nz]) < 1e-10, sweep
+        assert per_component_err(out[sweep][1], o.prim) < 1e-9, sweep
+    a, b = out["async"][0], out["levels"][0]
+    assert np.max(np.abs(a[nz] - b[nz]) / b[nz]) < 1e-12
+    assert per_component_err(out["async"][1], out["levels"][1]) < 1e-12
 
 
 def test_bind_rejects_bad_input():

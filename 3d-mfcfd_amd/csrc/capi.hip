@@ -77,7 +77,10 @@ uint64_t spread21(uint64_t v) {
     return v;
 }
 
-std::vector<int> morton_order(int n, const double* x, const double* y, const double* z) {
+// inst (may be null): instance of each point; points are ordered by
+// instance first, so every instance occupies one contiguous device range
+std::vector<int> morton_order(int n, const double* x, const double* y, const double* z,
+                              const int* inst = nullptr) {
     double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
     for (int i = 0; i < n; ++i) {
         const double v[3] = {x[i], y[i], z[i]};
@@ -86,7 +89,7 @@ std::vector<int> morton_order(int n, const double* x, const double* y, const dou
             hi[c] = std::max(hi[c], v[c]);
         }
     }
-    std::vector<std::pair<uint64_t, int>> key(n);
+    std::vector<std::pair<std::pair<int, uint64_t>, int>> key(n);
     for (int i = 0; i < n; ++i) {
         const double v[3] = {x[i], y[i], z[i]};
         uint64_t code = 0;
@@ -96,7 +99,7 @@ std::vector<int> morton_order(int n, const double* x, const double* y, const dou
             t = std::min(std::max(t, 0.0), 2097151.0);
             code |= spread21((uint64_t)t) << c;
         }
-        key[i] = {code, i};
+        key[i] = {{inst ? inst[i] : 0, code}, i};
     }
     std::sort(key.begin(), key.end());
     std::vector<int> perm(n);
@@ -126,6 +129,7 @@ struct mfcfd_ctx {
     TypeLists tl{};
     // state
     DevBuf<double> prim, U, q, qm, dq, delt, flux_res, delUp, delUn, dinv, temp, staging;
+    DevBuf<float> q32, qm32, dq32;   // mixed precision
     // residual
     DevBuf<double> partial;
     DevBuf<int> partial_pos;
@@ -141,12 +145,30 @@ struct mfcfd_ctx {
     DevBuf<int> fwd_seq, bwd_seq, fwd_dep, bwd_dep, sweep_flag;
     DevBuf<long long> fwd_dep_ptr, bwd_dep_ptr;
     DevBuf<unsigned> sweep_ctr;
-    int sweep_grid = 0;
+    // clouds of a batch (mfcfd_bind_batch): instance of each device point,
+    // freestream (u1_inf, u2_inf) per instance, device-order ranges; a single
+    // cloud is one instance with batch = false
+    bool batch = false;
+    int n_inst = 1;
+    DevBuf<int> inst_id, inst_beg, inst_end, hist_ctr;
+    DevBuf<double> inst_fs, inst_norm, ressq, inst_hist;
+    std::vector<int> inst_count;
+    // one iteration captured as a HIP graph (mfcfd_run_graph)
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_stale = true;
+    int sweep_grid = 0, sweep_grid_mixed = 0;
     bool implicit_supported = true;
     // bookkeeping of fpi_solver.f90:38-43
     double res_old = 0.0, res_new = 0.0, residue = 0.0, max_res = 0.0, sum_res_sqr = 0.0;
     int max_res_point = 0, last_t = 0;
     int n_regular = 0;
+    std::vector<int> host_class;   // per host point: 0 fused interior, 1 fused wall, 2 list-by-list
+    // HIP events around each fused interior flux launch (mfcfd_kernel_timing)
+    bool time_flux = false;
+    std::vector<hipEvent_t> tev;
+    size_t tev_used = 0;
+    double tev_ms = 0.0;
+    long long tev_launches = 0;
 
     DevGeom geom() const {
         DevGeom g;
@@ -164,6 +186,8 @@ struct mfcfd_ctx {
         g.alias = alias.p;
         g.respos = respos.p;
         g.perm = perm.p;
+        g.inst = batch ? inst_id.p : nullptr;
+        g.inst_fs = batch ? inst_fs.p : nullptr;
         return g;
     }
     DevState state() const {
@@ -178,6 +202,9 @@ struct mfcfd_ctx {
         s.delUp = delUp.p;
         s.delUn = delUn.p;
         s.dinv = dinv.p;
+        s.q32 = q32.p;
+        s.qm32 = qm32.p;
+        s.dq32 = dq32.p;
         return s;
     }
     void release_all() {
@@ -191,12 +218,21 @@ struct mfcfd_ctx {
         for (auto* b : ints) b->release();
         fwd_dep_ptr.release();
         bwd_dep_ptr.release();
+        q32.release();
+        qm32.release();
+        dq32.release();
         sweep_ctr.release();
+        DevBuf<int>* ib[] = {&inst_id, &inst_beg, &inst_end, &hist_ctr};
+        for (auto* b : ib) b->release();
+        DevBuf<double>* db[] = {&inst_fs, &inst_norm, &ressq, &inst_hist};
+        for (auto* b : db) b->release();
         dir_off.release();
         mask.release();
         pos.release();
     }
     ~mfcfd_ctx() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         release_all();
         rec.release();
         if (host_rec) (void)hipHostFree(host_rec);
@@ -226,6 +262,16 @@ struct ArrayDesc {
     int per_point;
 };
 
+// FP32 array the mixed form keeps for an array id (null: none)
+float* mixed_copy_of(mfcfd_ctx* c, int id) {
+    switch (id) {
+        case MFCFD_ARR_Q: return c->q32.p;
+        case MFCFD_ARR_QM: return c->qm32.p;
+        case MFCFD_ARR_DQ: return c->dq32.p;
+        default: return nullptr;
+    }
+}
+
 ArrayDesc array_of(mfcfd_ctx* c, int id) {
     switch (id) {
         case MFCFD_ARR_PRIM: return {&c->prim, 5};
@@ -248,10 +294,43 @@ int launch_error(const char* what) {
 }
 
 void flux(mfcfd_ctx* c, int parts) {
-    if (c->params.flux_order == MFCFD_FLUX_LISTS)
+    if (c->params.flux_order == MFCFD_FLUX_LISTS) {
         launch_flux_residual_reference(c->geom(), c->state(), c->fp, c->tl, c->stream);
-    else
-        launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream, parts);
+        return;
+    }
+    if (c->time_flux && (parts & 1)) {
+        if (c->tev_used + 2 > c->tev.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) == hipSuccess) {
+                if (hipEventCreate(&b) == hipSuccess) {
+                    c->tev.push_back(a);
+                    c->tev.push_back(b);
+                } else {
+                    (void)hipEventDestroy(a);
+                }
+            }
+        }
+        if (c->tev_used + 2 <= c->tev.size()) {
+            hipEventRecord(c->tev[c->tev_used], c->stream);
+            launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream, 1);
+            hipEventRecord(c->tev[c->tev_used + 1], c->stream);
+            c->tev_used += 2;
+            parts &= ~1;
+        }
+    }
+    if (parts) launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream, parts);
+}
+
+// after a stream synchronize: fold the recorded launch times into the totals
+void collect_kernel_times(mfcfd_ctx* c) {
+    for (size_t e = 0; e + 1 < c->tev_used; e += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->tev[e], c->tev[e + 1]) == hipSuccess) {
+            c->tev_ms += ms;
+            c->tev_launches += 1;
+        }
+    }
+    c->tev_used = 0;
 }
 
 int do_sweeps(mfcfd_ctx* c) {
@@ -287,20 +366,20 @@ int do_sweeps(mfcfd_ctx* c) {
     bwd.seq = c->bwd_seq.p;
     bwd.dep_ptr = c->bwd_dep_ptr.p;
     bwd.dep = c->bwd_dep.p;
-    launch_sweeps_async(g, s, c->fp, fwd, bwd, c->sweep_grid, c->stream);
+    launch_sweeps_async(g, s, c->fp, fwd, bwd, c->fp.mixed ? c->sweep_grid_mixed : c->sweep_grid, c->stream);
     return launch_error("implicit sweeps");
 }
 
-int do_state_update(mfcfd_ctx* c, double* hist, int hist_index) {
+int do_state_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
     launch_state_update(c->geom(), c->state(), c->fp, c->tl, c->partial.p, c->partial_pos.p, c->n_partial,
-                        c->rec.p, hist, hist_index, c->stream);
+                        c->rec.p, hist, hist_index, c->stream, ir);
     return launch_error("state update");
 }
 
-int do_iteration(mfcfd_ctx* c, double* hist, int hist_index) {
+int do_iteration(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
     const DevGeom g = c->geom();
     const DevState s = c->state();
-    launch_conserved_and_q(g, s, c->stream);
+    launch_conserved_and_q(g, s, c->fp, c->stream);
     if (c->fp.inner_iterations > 0) {
         launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
         launch_timestep(g, s, c->fp, c->stream);
@@ -311,7 +390,7 @@ int do_iteration(mfcfd_ctx* c, double* hist, int hist_index) {
     int rc = launch_error("iteration");
     if (rc) return rc;
     if ((rc = do_sweeps(c))) return rc;
-    return do_state_update(c, hist, hist_index);
+    return do_state_update(c, hist, hist_index, ir);
 }
 
 // fpi_solver.f90:38-43
@@ -329,6 +408,7 @@ void residue_bookkeeping(mfcfd_ctx* c, int t, double res_new) {
 int fetch_record(mfcfd_ctx* c) {
     HIP_TRY(hipMemcpyAsync(c->host_rec, c->rec.p, sizeof(ResidualRecord), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    collect_kernel_times(c);
     c->res_new = c->host_rec->res_new;
     c->sum_res_sqr = c->host_rec->sum_res_sqr;
     c->max_res = c->host_rec->max_res;
@@ -399,7 +479,14 @@ int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
     if (p->mode != MFCFD_EXPLICIT && p->mode != MFCFD_IMPLICIT) return fail(MFCFD_EINVAL, "mode %d", p->mode);
     if (p->precision != MFCFD_FP64 && p->precision != MFCFD_MIXED)
         return fail(MFCFD_EINVAL, "precision %d", p->precision);
-    if (p->precision == MFCFD_MIXED) return fail(MFCFD_EUNSUPPORTED, "mixed precision is not built in");
+    if (p->precision == MFCFD_MIXED) {
+        if (p->flux_order != MFCFD_FLUX_FUSED)
+            return fail(MFCFD_EUNSUPPORTED, "mixed precision runs with the fused flux order only");
+        if (p->sweep_schedule != MFCFD_SWEEP_ASYNC)
+            return fail(MFCFD_EUNSUPPORTED, "mixed precision runs with the sync-free sweeps only");
+        if (p->inner_iterations > 0)
+            return fail(MFCFD_EUNSUPPORTED, "mixed precision has no second-order inner iterations");
+    }
     if (p->flux_order != MFCFD_FLUX_FUSED && p->flux_order != MFCFD_FLUX_LISTS)
         return fail(MFCFD_EINVAL, "flux_order %d", p->flux_order);
     if (p->sweep_schedule != MFCFD_SWEEP_ASYNC && p->sweep_schedule != MFCFD_SWEEP_LEVELS)
@@ -422,7 +509,9 @@ int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
     f.u3_inf = 0.0;
     f.pr_inf = 1.0 / 1.4;
     f.inner_iterations = p->inner_iterations;
+    f.mixed = p->precision == MFCFD_MIXED ? 1 : 0;
     c->have_params = true;
+    c->graph_stale = true;
     if (c->have_geom && p->inner_iterations > 0 && c->temp.n == 0) {
         if (c->temp.alloc(15ULL * c->n) != hipSuccess) return fail(MFCFD_ENOMEM, "temp(3,5,N)");
     }
@@ -457,9 +546,12 @@ int mfcfd_aliasing(int n, const int* nbhs, const int* conn, int* alias, int* poi
     return MFCFD_OK;
 }
 
-int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
-    int rc = check_ctx(c, false);
-    if (rc) return rc;
+namespace {
+
+// inst_of (host order) and fs (u1_inf, u2_inf per instance) describe a
+// batch; both null for a single cloud
+int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_inst, const double* fs) {
+    int rc = 0;
     if (!gm || gm->n <= 0) return fail(MFCFD_EINVAL, "empty geometry");
     const int n = gm->n;
     const double* dp[] = {gm->x, gm->y, gm->z, gm->tan1, gm->tan2, gm->nor, gm->min_dist};
@@ -504,7 +596,7 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
     }
 
     // ---- device order ------------------------------------------------
-    std::vector<int> perm = morton_order(n, gm->x, gm->y, gm->z);  // device -> host
+    std::vector<int> perm = morton_order(n, gm->x, gm->y, gm->z, inst_of);  // device -> host
     std::vector<int> inv(n);
     for (int d = 0; d < n; ++d) inv[perm[d]] = d;
 
@@ -634,6 +726,9 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
     c->tl.outer = c->idx_outer.p;
     c->tl.sup_in = c->idx_sup_in.p;
     c->tl.sup_out = c->idx_sup_out.p;
+    c->host_class.assign(n, 2);
+    for (int d : lfi) c->host_class[perm[d]] = 0;
+    for (int d : lfw) c->host_class[perm[d]] = 1;
     c->tl.n_fused_interior = (int)lfi.size();
     c->tl.n_fused_wall = (int)lfw.size();
     c->tl.n_generic = (int)lgen.size();
@@ -657,6 +752,12 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
         return fail(MFCFD_ENOMEM, "state arrays");
     HIP_TRY(hipMemsetAsync(c->qm.p, 0, 10ULL * n * sizeof(double), st));
     HIP_TRY(hipMemsetAsync(c->dq.p, 0, 15ULL * n * sizeof(double), st));
+    if (c->q32.alloc(5ULL * n) != hipSuccess || c->qm32.alloc(10ULL * n) != hipSuccess ||
+        c->dq32.alloc(15ULL * n) != hipSuccess)
+        return fail(MFCFD_ENOMEM, "FP32 q arrays");
+    HIP_TRY(hipMemsetAsync(c->q32.p, 0, 5ULL * n * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->qm32.p, 0, 10ULL * n * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->dq32.p, 0, 15ULL * n * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->delt.p, 0, (size_t)n * sizeof(double), st));
     HIP_TRY(hipMemsetAsync(c->dinv.p, 0, (size_t)n * sizeof(double), st));
     if (c->fp.inner_iterations > 0 && c->temp.alloc(15ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "temp");
@@ -725,12 +826,153 @@ int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
         HIP_TRY(hipMemsetAsync(c->sweep_flag.p, 0, (size_t)n * sizeof(int), st));
         HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, 2 * sizeof(unsigned), st));
         HIP_TRY(hipMemsetAsync(c->rec.p, 0, sizeof(ResidualRecord), st));
-        c->sweep_grid = sweep_async_grid((int)m, c->params.power == 0.0);
+        c->sweep_grid = sweep_async_grid((int)m, c->params.power == 0.0, false);
+        c->sweep_grid_mixed = sweep_async_grid((int)m, c->params.power == 0.0, true);
+    }
+    // instances: device-order ranges (one for a single cloud), per-point
+    // instance and freestream table for a batch
+    {
+        c->batch = inst_of != nullptr;
+        c->n_inst = c->batch ? n_inst : 1;
+        std::vector<int> beg(c->n_inst, n), end(c->n_inst, 0), id(n, 0);
+        for (int d = 0; d < n; ++d) {
+            const int b = c->batch ? inst_of[perm[d]] : 0;
+            id[d] = b;
+            beg[b] = std::min(beg[b], d);
+            end[b] = std::max(end[b], d + 1);
+        }
+        c->inst_count.assign(c->n_inst, 0);
+        std::vector<double> norm(c->n_inst);
+        for (int b = 0; b < c->n_inst; ++b) {
+            if (end[b] <= beg[b]) beg[b] = end[b] = 0;
+            c->inst_count[b] = end[b] - beg[b];
+            norm[b] = (double)c->inst_count[b];
+        }
+        if ((rc = upload(c->inst_beg, beg.data(), beg.size(), st))) return rc;
+        if ((rc = upload(c->inst_end, end.data(), end.size(), st))) return rc;
+        if ((rc = upload(c->inst_norm, norm.data(), norm.size(), st))) return rc;
+        if (c->batch) {
+            if ((rc = upload(c->inst_id, id.data(), id.size(), st))) return rc;
+            if ((rc = upload(c->inst_fs, fs, 2ULL * n_inst, st))) return rc;
+        }
+        if (c->ressq.alloc((size_t)n) != hipSuccess || c->hist_ctr.alloc(1) != hipSuccess)
+            return fail(MFCFD_ENOMEM, "instance residual buffers");
+        HIP_TRY(hipMemsetAsync(c->ressq.p, 0, (size_t)n * sizeof(double), st));
+        HIP_TRY(hipMemsetAsync(c->hist_ctr.p, 0, sizeof(int), st));
     }
     HIP_TRY(hipStreamSynchronize(st));
     c->have_geom = true;
+    c->graph_stale = true;
     c->res_old = c->res_new = c->residue = 0.0;
     c->last_t = 0;
+    return MFCFD_OK;
+}
+
+}  // namespace
+
+int mfcfd_bind_geometry(mfcfd_ctx* c, const mfcfd_geometry* gm) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    return bind_impl(c, gm, nullptr, 1, nullptr);
+}
+
+int mfcfd_bind_batch(mfcfd_ctx* c, int n_inst, const mfcfd_geometry* geoms, const double* mach,
+                     const double* aoa) {
+    int rc = check_ctx(c, false);
+    if (rc) return rc;
+    if (n_inst <= 0 || !geoms || !mach || !aoa) return fail(MFCFD_EINVAL, "empty batch");
+    long long total = 0;
+    for (int b = 0; b < n_inst; ++b) {
+        const mfcfd_geometry* g = geoms + b;
+        if (g->n <= 0 || !g->x || !g->y || !g->z || !g->tan1 || !g->tan2 || !g->nor || !g->min_dist ||
+            !g->status || !g->nbhs || !g->conn || !g->alias || !g->point_with_alias)
+            return fail(MFCFD_EINVAL, "instance %d: null or empty geometry", b);
+        for (int d = 0; d < 6; ++d)
+            if (!g->split_nbhs[d] || !g->split_conn[d]) return fail(MFCFD_EINVAL, "instance %d: null split", b);
+        for (int i = 0; i < g->n; ++i) {
+            if (g->status[i] == 5 || g->status[i] == 6)
+                return fail(MFCFD_EUNSUPPORTED, "instance %d: supersonic points in a batch", b);
+            // a list that spills past the instance's last point would land in
+            // the next instance's columns
+            for (int d = 0; d < 6; ++d) {
+                const HostList hl = host_list(g, i, d);
+                if (hl.p && hl.p != g->conn + 27LL * i && 22LL * i + hl.count > 22LL * g->n)
+                    return fail(MFCFD_EINVAL, "instance %d point %d: split list %d runs past the instance", b,
+                                i + 1, d);
+            }
+        }
+        total += g->n;
+    }
+    if (total > 0x7fffffffLL / 27) return fail(MFCFD_EINVAL, "batch too large");
+    const int n = (int)total;
+    std::vector<double> x(n), y(n), z(n), t1(3ULL * n), t2(3ULL * n), nr(3ULL * n), md(n);
+    std::vector<int> status(n), nbhs(n), conn(27ULL * n, 0), alias(n), pwa(n), inst(n);
+    std::vector<int> scnt[6], sconn[6];
+    for (int d = 0; d < 6; ++d) {
+        scnt[d].assign(n, 0);
+        sconn[d].assign(22ULL * n, 0);
+    }
+    std::vector<double> fs(2ULL * n_inst);
+    const double pi = 4.0 * std::atan(1.0);
+    long long off = 0;
+    for (int b = 0; b < n_inst; ++b) {
+        const mfcfd_geometry* g = geoms + b;
+        const int m = g->n;
+        const int o = (int)off;
+        auto shift = [o](int v) { return v > 0 ? v + o : 0; };
+        for (int i = 0; i < m; ++i) {
+            x[o + i] = g->x[i];
+            y[o + i] = g->y[i];
+            z[o + i] = g->z[i];
+            md[o + i] = g->min_dist[i];
+            for (int cc = 0; cc < 3; ++cc) {
+                t1[3LL * (o + i) + cc] = g->tan1[3LL * i + cc];
+                t2[3LL * (o + i) + cc] = g->tan2[3LL * i + cc];
+                nr[3LL * (o + i) + cc] = g->nor[3LL * i + cc];
+            }
+            status[o + i] = g->status[i];
+            nbhs[o + i] = g->nbhs[i];
+            for (int j = 0; j < 27; ++j) conn[27LL * (o + i) + j] = shift(g->conn[27LL * i + j]);
+            alias[o + i] = g->alias[i] > 0 ? g->alias[i] + o : 0;
+            pwa[o + i] = shift(g->point_with_alias[i]);
+            inst[o + i] = b;
+            for (int d = 0; d < 6; ++d) {
+                scnt[d][o + i] = g->split_nbhs[d][i];
+                for (int j = 0; j < 22; ++j) sconn[d][22LL * (o + i) + j] = shift(g->split_conn[d][22LL * i + j]);
+            }
+        }
+        // parameter.f90:29-31 with the instance's Mach and angle of attack
+        const double theta = aoa[b] * pi / 180.0;
+        fs[2 * b] = mach[b] * std::cos(theta);
+        fs[2 * b + 1] = mach[b] * std::sin(theta);
+        off += m;
+    }
+    mfcfd_geometry mg;
+    mg.n = n;
+    mg.x = x.data();
+    mg.y = y.data();
+    mg.z = z.data();
+    mg.tan1 = t1.data();
+    mg.tan2 = t2.data();
+    mg.nor = nr.data();
+    mg.min_dist = md.data();
+    mg.status = status.data();
+    mg.nbhs = nbhs.data();
+    mg.conn = conn.data();
+    for (int d = 0; d < 6; ++d) {
+        mg.split_nbhs[d] = scnt[d].data();
+        mg.split_conn[d] = sconn[d].data();
+    }
+    mg.alias = alias.data();
+    mg.point_with_alias = pwa.data();
+    return bind_impl(c, &mg, inst.data(), n_inst, fs.data());
+}
+
+int mfcfd_batch_info(mfcfd_ctx* c, int* n_inst, int* counts) {
+    if (!c || !c->have_geom) return fail(MFCFD_ESTATE, "no geometry bound");
+    if (n_inst) *n_inst = c->n_inst;
+    if (counts)
+        for (int b = 0; b < c->n_inst; ++b) counts[b] = c->inst_count[b];
     return MFCFD_OK;
 }
 
@@ -773,6 +1015,9 @@ int mfcfd_upload(mfcfd_ctx* c, int id, const double* host) {
     const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
     HIP_TRY(hipMemcpyAsync(c->staging.p, host, bytes, hipMemcpyHostToDevice, c->stream));
     launch_gather(a.buf->p, c->staging.p, c->perm.p, c->n, a.per_point, c->stream);
+    // mixed form: the double array is staging for the FP32 copy the kernels read
+    if (float* f32 = c->fp.mixed ? mixed_copy_of(c, id) : nullptr)
+        launch_to_float(f32, a.buf->p, (long long)a.per_point * c->n, c->stream);
     if ((rc = launch_error("upload permutation"))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MFCFD_OK;
@@ -784,6 +1029,8 @@ int mfcfd_download(mfcfd_ctx* c, int id, double* host) {
     ArrayDesc a = array_of(c, id);
     if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
     const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
+    if (float* f32 = c->fp.mixed ? mixed_copy_of(c, id) : nullptr)
+        launch_to_double(a.buf->p, f32, (long long)a.per_point * c->n, c->stream);
     launch_scatter(c->staging.p, a.buf->p, c->perm.p, c->n, a.per_point, c->stream);
     if ((rc = launch_error("download permutation"))) return rc;
     HIP_TRY(hipMemcpyAsync(host, c->staging.p, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -801,7 +1048,7 @@ int mfcfd_compute_conserved_vector(mfcfd_ctx* c) {
 int mfcfd_eval_q_variables(mfcfd_ctx* c) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
-    launch_q_variables(c->geom(), c->state(), c->stream);
+    launch_q_variables(c->geom(), c->state(), c->fp, c->stream);
     return launch_error("eval_q_variables");
 }
 
@@ -855,8 +1102,10 @@ int mfcfd_run(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, double* r
     if (rc) return rc;
     if (n_iters < 0) return fail(MFCFD_EINVAL, "n_iters < 0");
     if (n_iters == 0) return MFCFD_OK;
-    if (c->hist.n < (size_t)n_iters && c->hist.alloc((size_t)n_iters) != hipSuccess)
-        return fail(MFCFD_ENOMEM, "history buffer");
+    if (c->hist.n < (size_t)n_iters) {
+        if (c->hist.alloc((size_t)n_iters) != hipSuccess) return fail(MFCFD_ENOMEM, "history buffer");
+        c->graph_stale = true;
+    }
     for (int it = 0; it < n_iters; ++it)
         if ((rc = do_iteration(c, c->hist.p, it))) return rc;
     std::vector<double> h(n_iters);
@@ -866,6 +1115,66 @@ int mfcfd_run(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, double* r
         residue_bookkeeping(c, t0 + it, h[it]);
         if (res_new_hist) res_new_hist[it] = c->res_new;
         if (residue_hist) residue_hist[it] = c->residue;
+    }
+    return MFCFD_OK;
+}
+
+int mfcfd_run_graph(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, double* inst_res_hist) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (n_iters < 0) return fail(MFCFD_EINVAL, "n_iters < 0");
+    if (n_iters == 0) return MFCFD_OK;
+    if (c->time_flux) return fail(MFCFD_ESTATE, "kernel timing records events; disable it before graph runs");
+    // the history buffers' addresses are captured: grow them before capture
+    const size_t need = (size_t)std::max(n_iters, 64);
+    if (c->hist.n < need) {
+        if (c->hist.alloc(2 * need) != hipSuccess) return fail(MFCFD_ENOMEM, "history buffer");
+        c->graph_stale = true;
+    }
+    if (c->inst_hist.n < c->hist.n * c->n_inst) {
+        if (c->inst_hist.alloc(c->hist.n * c->n_inst) != hipSuccess)
+            return fail(MFCFD_ENOMEM, "instance history buffer");
+        c->graph_stale = true;
+    }
+    InstanceResidual ir;
+    ir.ressq = c->ressq.p;
+    ir.beg = c->inst_beg.p;
+    ir.end = c->inst_end.p;
+    ir.norm = c->inst_norm.p;
+    ir.n_inst = c->n_inst;
+    ir.inst_hist = c->inst_hist.p;
+    ir.counter = c->hist_ctr.p;
+    if (c->graph_stale || !c->graph_exec) {
+        if (c->graph_exec) {
+            (void)hipGraphExecDestroy(c->graph_exec);
+            c->graph_exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = do_iteration(c, c->hist.p, 0, &ir);
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(MFCFD_EHIP, "graph capture: %s", hipGetErrorString(e));
+        const hipError_t ie = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail(MFCFD_EHIP, "graph instantiate: %s", hipGetErrorString(ie));
+        c->graph_stale = false;
+    }
+    // history rows restart at 0 for this call (outside the graph)
+    HIP_TRY(hipMemsetAsync(c->hist_ctr.p, 0, sizeof(int), c->stream));
+    for (int it = 0; it < n_iters; ++it) HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+    std::vector<double> h(n_iters);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->hist.p, sizeof(double) * n_iters, hipMemcpyDeviceToHost, c->stream));
+    if (inst_res_hist)
+        HIP_TRY(hipMemcpyAsync(inst_res_hist, c->inst_hist.p, sizeof(double) * n_iters * c->n_inst,
+                               hipMemcpyDeviceToHost, c->stream));
+    if ((rc = fetch_record(c))) return rc;
+    for (int it = 0; it < n_iters; ++it) {
+        residue_bookkeeping(c, t0 + it, h[it]);
+        if (res_new_hist) res_new_hist[it] = c->res_new;
     }
     return MFCFD_OK;
 }
@@ -899,7 +1208,7 @@ int mfcfd_time_phases(mfcfd_ctx* c, int t0, int n_iters, double* out_ms) {
     const DevState s = c->state();
     for (int it = 0; it < n_iters && rc == 0; ++it) {
         hipEventRecord(ev[0], c->stream);
-        launch_conserved_and_q(g, s, c->stream);
+        launch_conserved_and_q(g, s, c->fp, c->stream);
         hipEventRecord(ev[1], c->stream);
         if (c->fp.inner_iterations > 0) {
             launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
@@ -945,6 +1254,32 @@ int mfcfd_sweep_info(mfcfd_ctx* c, int* levels_fwd, int* levels_bwd, int* n_swep
     if (levels_fwd) *levels_fwd = c->sched.n_levels_fwd;
     if (levels_bwd) *levels_bwd = c->sched.n_levels_bwd;
     if (n_swept) *n_swept = (int)c->sched.order.size();
+    return MFCFD_OK;
+}
+
+int mfcfd_point_class_map(mfcfd_ctx* c, int* out) {
+    if (!c || !c->have_geom) return fail(MFCFD_ESTATE, "no geometry bound");
+    if (!out) return fail(MFCFD_EINVAL, "null argument");
+    std::memcpy(out, c->host_class.data(), sizeof(int) * c->host_class.size());
+    return MFCFD_OK;
+}
+
+int mfcfd_kernel_timing(mfcfd_ctx* c, int enable) {
+    if (!c) return fail(MFCFD_EINVAL, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->time_flux = enable != 0;
+    c->tev_used = 0;
+    c->tev_ms = 0.0;
+    c->tev_launches = 0;
+    return MFCFD_OK;
+}
+
+int mfcfd_kernel_time(mfcfd_ctx* c, double* total_ms, long long* launches) {
+    if (!c) return fail(MFCFD_EINVAL, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    collect_kernel_times(c);
+    if (total_ms) *total_ms = c->tev_ms;
+    if (launches) *launches = c->tev_launches;
     return MFCFD_OK;
 }
 

@@ -12,6 +12,18 @@ constexpr int kBlock = 256;
 
 inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
 
+template <typename T>
+__device__ __forceinline__ Frame<T> frame_as(const Frame<double>& f) {
+    Frame<T> o;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        o.t1[c] = T(f.t1[c]);
+        o.t2[c] = T(f.t2[c]);
+        o.n[c] = T(f.n[c]);
+    }
+    return o;
+}
+
 __device__ __forceinline__ Frame<double> load_frame(const double* frame, int k) {
     Frame<double> f;
     const double* p = frame + 9 * (long long)k;
@@ -35,19 +47,19 @@ __device__ __forceinline__ double wls_weight(double a, double b, double c, doubl
 // Limited reconstruction of q at one end of an edge and conversion to
 // primitive variables (interior_flux_dGxpos.f90:92-96,99-103):
 //   qtilde = q - 0.5*phi*(delx*dq1 + dely*dq2 + delz*dq3)
-__device__ __forceinline__ void reconstruct_end(const double q[5], const double* dq, const double* qm,
-                                                double epsi2, double delx, double dely, double delz,
-                                                double prim[5]) {
-    double temp[5], qt[5];
+template <typename T>
+__device__ __forceinline__ void reconstruct_end(const T q[5], const T* dq, const T* qm, T epsi2, T delx,
+                                                T dely, T delz, T prim[5]) {
+    T temp[5], qt[5];
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
         temp[r] = delx * dq[3 * r] + dely * dq[3 * r + 1] + delz * dq[3 * r + 2];
-        qt[r] = q[r] - 0.5 * temp[r];
+        qt[r] = q[r] - T(0.5) * temp[r];
     }
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
-        const double phi = venkat_phi(qt[r], q[r], qm[2 * r], qm[2 * r + 1], epsi2);
-        qt[r] = q[r] - 0.5 * phi * temp[r];
+        const T phi = venkat_phi(qt[r], q[r], qm[2 * r], qm[2 * r + 1], epsi2);
+        qt[r] = q[r] - T(0.5) * phi * temp[r];
     }
     qtilde_to_primitive(qt, prim);
 }

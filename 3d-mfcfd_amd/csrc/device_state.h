@@ -52,6 +52,10 @@ struct DevGeom {
     const int* alias;
     const int* respos;           // position in the residual order (wall, then interior)
     const int* perm;             // device point -> host point (0-based)
+    // batches of clouds: instance of each point and each instance's
+    // freestream velocity (u1_inf, u2_inf); null for a single cloud
+    const int* inst;
+    const double* inst_fs;
 };
 
 struct DevState {
@@ -65,12 +69,36 @@ struct DevState {
     double* delUp;
     double* delUn;
     double* dinv;
+    // mixed precision: q, qm, dq held in FP32 (same layouts)
+    float* q32;
+    float* qm32;
+    float* dq32;
 };
+
+// q-variable arrays of one precision
+template <typename T>
+struct QArrays {
+    T* q;
+    T* qm;
+    T* dq;
+};
+
+template <typename T>
+__host__ __device__ inline QArrays<T> qarrays(const DevState& s);
+template <>
+__host__ __device__ inline QArrays<double> qarrays<double>(const DevState& s) {
+    return {s.q, s.qm, s.dq};
+}
+template <>
+__host__ __device__ inline QArrays<float> qarrays<float>(const DevState& s) {
+    return {s.q32, s.qm32, s.dq32};
+}
 
 struct FlowParams {
     double cfl, vl_const, power, pi;
     double rho_inf, u1_inf, u2_inf, u3_inf, pr_inf;
     int inner_iterations;
+    int mixed;   // 1: q, qm, dq and the split fluxes in FP32
 };
 
 // device-side residual record, state_update.f90:115 and data_structure.f90:58-59

@@ -47,12 +47,15 @@ struct SweepAsync {
 void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, hipStream_t st);
 // boundary permutations: dst[k*m + c] = src[perm[k]*m + c] and the inverse
 void launch_gather(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
+// precision conversions of a flat array (mixed form uploads and downloads)
+void launch_to_float(float* dst, const double* src, long long n, hipStream_t st);
+void launch_to_double(double* dst, const float* src, long long n, hipStream_t st);
 void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
 
 // compute_conserved_vector.f90 / q_variables.f90
 void launch_conserved_vector(const DevGeom& g, const DevState& s, hipStream_t st);
-void launch_q_variables(const DevGeom& g, const DevState& s, hipStream_t st);
-void launch_conserved_and_q(const DevGeom& g, const DevState& s, hipStream_t st);
+void launch_q_variables(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
+void launch_conserved_and_q(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
 // q_derivatives.f90 / timestep_delt.f90
 void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams& p, double* temp,
                           hipStream_t st);
@@ -76,11 +79,24 @@ void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams&
 void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
                          const SweepAsync& bwd, int grid, hipStream_t st);
 // resident workgroup count used for the sync-free sweeps
-int sweep_async_grid(int count, bool unit_weight);
+int sweep_async_grid(int count, bool unit_weight, bool mixed);
 // state_update.f90 (+ residual reduction into *rec; hist may be null)
+// per-cloud residuals of a batch, recorded in row *counter of inst_hist
+// (n_inst per row); the counter advances once per state update
+struct InstanceResidual {
+    double* ressq;       // [n] per-point squared density change
+    const int* beg;      // [n_inst] device-order point range of each cloud
+    const int* end;
+    const double* norm;  // [n_inst] the cloud's point count
+    int n_inst;
+    double* inst_hist;
+    int* counter;
+};
+// hist[hist_index] (or hist[*ir->counter] when ir is given) receives res_new
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p,
                          const TypeLists& tl, double* partial, int* partial_pos, int n_partial,
-                         ResidualRecord* rec, double* hist, int hist_index, hipStream_t st);
+                         ResidualRecord* rec, double* hist, int hist_index, hipStream_t st,
+                         const InstanceResidual* ir = nullptr);
 int state_update_blocks(int n);
 
 }  // namespace mfcfd

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(kBlock) k_stencil_coef(DevGeom g, double* coef
 // ---------------------------------------------------------------------------
 // compute_conserved_vector.f90:20-32 and q_variables.f90:18-38
 // ---------------------------------------------------------------------------
-template <bool DO_U, bool DO_Q>
+template <bool DO_U, bool DO_Q, typename QT>
 __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= g.n) return;
@@ -115,8 +115,9 @@ __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
         q[2] = two_times_beta * u2;
         q[3] = two_times_beta * u3;
         q[4] = -two_times_beta;
+        QT* qo = qarrays<QT>(s).q;
 #pragma unroll
-        for (int r = 0; r < 5; ++r) s.q[5 * (long long)k + r] = q[r];
+        for (int r = 0; r < 5; ++r) qo[5 * (long long)k + r] = QT(q[r]);
     }
 }
 
@@ -126,23 +127,27 @@ __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
 #ifndef MFCFD_QDERIV_WAVES
 #define MFCFD_QDERIV_WAVES 4
 #endif
-template <bool DO_DQ, bool DO_DT, bool UW>
+// QT: type of q, qm, dq and of the gradient right-hand sides (double: the
+// FP64 form; float: the mixed form); the geometric sums and the 3x3 solve
+// stay FP64
+template <bool DO_DQ, bool DO_DT, bool UW, typename QT>
 __global__ void __launch_bounds__(kBlock, MFCFD_QDERIV_WAVES) k_qderiv_timestep(DevGeom g, DevState s, FlowParams fp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
+    const QArrays<QT> qa = qarrays<QT>(s);
     const double4 pi4 = g.pos[i];
     const int nb = g.nbhs[i];
     const int* cn = g.conn + kConnCols * (long long)i;
-    double qi[5], qmax[5], qmin[5];
-    double sx[5], sy[5], sz[5];
+    QT qi[5], qmax[5], qmin[5];
+    QT sx[5], sy[5], sz[5];
     Geo6<double> geo = {0, 0, 0, 0, 0, 0};
     if (DO_DQ) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            qi[r] = s.q[5 * (long long)i + r];
+            qi[r] = qa.q[5 * (long long)i + r];
             qmax[r] = qi[r];
             qmin[r] = qi[r];
-            sx[r] = sy[r] = sz[r] = 0.0;
+            sx[r] = sy[r] = sz[r] = QT(0);
         }
     }
     double min_delt = 1.0;
@@ -153,9 +158,9 @@ __global__ void __launch_bounds__(kBlock, MFCFD_QDERIV_WAVES) k_qderiv_timestep(
         const double dely = pk.y - pi4.y;
         const double delz = pk.z - pi4.z;
         if (DO_DQ) {
-            double qk[5];
+            QT qk[5];
 #pragma unroll
-            for (int r = 0; r < 5; ++r) qk[r] = s.q[5 * (long long)k + r];
+            for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 if (qk[r] > qmax[r]) qmax[r] = qk[r];
@@ -168,12 +173,13 @@ __global__ void __launch_bounds__(kBlock, MFCFD_QDERIV_WAVES) k_qderiv_timestep(
             geo.xy = geo.xy + delx * dely * weights;
             geo.yz = geo.yz + dely * delz * weights;
             geo.zx = geo.zx + delz * delx * weights;
+            const QT wx = QT(weights * delx), wy = QT(weights * dely), wz = QT(weights * delz);
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
-                const double t = qk[r] - qi[r];
-                sx[r] = sx[r] + weights * delx * t;
-                sy[r] = sy[r] + weights * dely * t;
-                sz[r] = sz[r] + weights * delz * t;
+                const QT t = qk[r] - qi[r];
+                sx[r] = sx[r] + wx * t;
+                sy[r] = sy[r] + wy * t;
+                sz[r] = sz[r] + wz * t;
             }
         }
         if (DO_DT) {
@@ -190,11 +196,12 @@ __global__ void __launch_bounds__(kBlock, MFCFD_QDERIV_WAVES) k_qderiv_timestep(
         const double one_by_det = 1.0 / det;
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            s.qm[(5 * (long long)i + r) * 2 + 0] = qmax[r];
-            s.qm[(5 * (long long)i + r) * 2 + 1] = qmin[r];
+            qa.qm[(5 * (long long)i + r) * 2 + 0] = qmax[r];
+            qa.qm[(5 * (long long)i + r) * 2 + 1] = qmin[r];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                s.dq[(5 * (long long)i + r) * 3 + c] = cramer_num(geo, c, sx[r], sy[r], sz[r]) * one_by_det;
+                qa.dq[(5 * (long long)i + r) * 3 + c] =
+                    QT(cramer_num(geo, c, (double)sx[r], (double)sy[r], (double)sz[r]) * one_by_det);
         }
     }
     if (DO_DT) s.delt[i] = min_delt;
@@ -245,6 +252,16 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_inner(DevGeom g, DevState s, 
             temp[(5 * (long long)i + r) * 3 + c] = cramer_num(geo, c, sx[r], sy[r], sz[r]) * one_by_det;
 }
 
+__global__ void k_to_float(float* dst, const double* src, long long n) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+__global__ void k_to_double(double* dst, const float* src, long long n) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i];
+}
+
 __global__ void k_copy(double* dst, const double* src, long long n) {
     const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
@@ -274,16 +291,20 @@ __device__ __forceinline__ Stencil dir_stencil(const DevGeom& g, int cls, int d,
     return st;
 }
 
-template <bool UW>
+// T: arithmetic of the reconstruction and fluxes (double: FP64 form; float:
+// mixed form); the least-squares sums and the 3x3 solve stay FP64
+template <bool UW, typename T>
 __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
                            const Stencil& st, const Frame<double>& f, double G[5]) {
+    const QArrays<T> qa = qarrays<T>(s);
+    const Frame<T> ft = frame_as<T>(f);
     Geo6<double> geo = {0, 0, 0, 0, 0, 0};
     double sx[5] = {0, 0, 0, 0, 0}, sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0};
     const double4 pi4 = g.pos[i];
-    double qi[5];
+    T qi[5];
 #pragma unroll
-    for (int r = 0; r < 5; ++r) qi[r] = s.q[5 * (long long)i + r];
-    const double epsi2_i = limiter_epsi2(fp.vl_const, pi4.w);
+    for (int r = 0; r < 5; ++r) qi[r] = qa.q[5 * (long long)i + r];
+    const T epsi2_i = T(limiter_epsi2(fp.vl_const, pi4.w));
     for (int j = 0; j < st.count; ++j) {
         const int k = st.list[j];
         const double4 pk = g.pos[k];
@@ -301,18 +322,18 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
         geo.xy = geo.xy + dels * delt_w;
         geo.yz = geo.yz + delt * deln_w;
         geo.zx = geo.zx + deln * dels_w;
-        double prim[5], Gi[5], Gk[5], qk[5];
-        reconstruct_end(qi, s.dq + 15 * (long long)i, s.qm + 10 * (long long)i, epsi2_i, delx, dely, delz,
-                        prim);
-        flux_eval(st.family, st.axis, st.sign, f, prim, fp.pi, Gi);
+        T prim[5], Gi[5], Gk[5], qk[5];
+        reconstruct_end<T>(qi, qa.dq + 15 * (long long)i, qa.qm + 10 * (long long)i, epsi2_i, T(delx), T(dely),
+                           T(delz), prim);
+        flux_eval<T>(st.family, st.axis, st.sign, ft, prim, T(fp.pi), Gi);
 #pragma unroll
-        for (int r = 0; r < 5; ++r) qk[r] = s.q[5 * (long long)k + r];
-        reconstruct_end(qk, s.dq + 15 * (long long)k, s.qm + 10 * (long long)k,
-                        limiter_epsi2(fp.vl_const, pk.w), delx, dely, delz, prim);
-        flux_eval(st.family, st.axis, st.sign, f, prim, fp.pi, Gk);
+        for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
+        reconstruct_end<T>(qk, qa.dq + 15 * (long long)k, qa.qm + 10 * (long long)k,
+                           T(limiter_epsi2(fp.vl_const, pk.w)), T(delx), T(dely), T(delz), prim);
+        flux_eval<T>(st.family, st.axis, st.sign, ft, prim, T(fp.pi), Gk);
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            const double t = Gk[r] - Gi[r];
+            const double t = (double)(Gk[r] - Gi[r]);
             sx[r] = sx[r] + t * dels_w;
             sy[r] = sy[r] + t * delt_w;
             sz[r] = sz[r] + t * deln_w;
@@ -330,7 +351,7 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
 }
 
 // flux_residual.f90:40-81 for the points of idx (any class, from status)
-template <bool UW>
+template <bool UW, typename T>
 __global__ void __launch_bounds__(kBlock) k_flux_residual_lists(DevGeom g, DevState s, FlowParams fp,
                                                                 const int* idx, int count) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -339,20 +360,20 @@ __global__ void __launch_bounds__(kBlock) k_flux_residual_lists(DevGeom g, DevSt
     const int cls = g.status[i];
     const Frame<double> f = load_frame(g.frame, i);
     double acc[5], G[5];
-    stencil_dG<UW>(g, s, fp, i, dir_stencil(g, cls, XPOS, i), f, acc);
+    stencil_dG<UW, T>(g, s, fp, i, dir_stencil(g, cls, XPOS, i), f, acc);
     const int last = cls == 0 ? ZNEG : YNEG;
 #pragma unroll 1
     for (int d = XNEG; d <= last; ++d) {
-        stencil_dG<UW>(g, s, fp, i, dir_stencil(g, cls, d, i), f, G);
+        stencil_dG<UW, T>(g, s, fp, i, dir_stencil(g, cls, d, i), f, G);
 #pragma unroll
         for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
     }
     if (cls == 1) {
-        stencil_dG<UW>(g, s, fp, i, dir_stencil(g, 1, ZNEG, i), f, G);
+        stencil_dG<UW, T>(g, s, fp, i, dir_stencil(g, 1, ZNEG, i), f, G);
 #pragma unroll
         for (int r = 0; r < 5; ++r) acc[r] = 2.0 * (acc[r] + G[r]);
     } else if (cls == 2) {
-        stencil_dG<UW>(g, s, fp, i, dir_stencil(g, 2, ZPOS, i), f, G);
+        stencil_dG<UW, T>(g, s, fp, i, dir_stencil(g, 2, ZPOS, i), f, G);
 #pragma unroll
         for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
     }
@@ -542,9 +563,14 @@ __global__ void __launch_bounds__(64) k_sweep_level(DevGeom g, DevState s, FlowP
 // ---------------------------------------------------------------------------
 // state_update.f90:22-115 with the residual of state_update.f90:39-44,115
 // ---------------------------------------------------------------------------
-__device__ void conserved_vector_Ubar(const DevState& s, const FlowParams& fp, int k,
+__device__ void conserved_vector_Ubar(const DevGeom& g, const DevState& s, FlowParams fp, int k,
                                       const Frame<double>& f, double Ubar[5]) {
-    // conserved_vector_Ubar.f90:19-58
+    // conserved_vector_Ubar.f90:19-58; a point of a batch takes its cloud's freestream
+    if (g.inst) {
+        const int b = g.inst[k];
+        fp.u1_inf = g.inst_fs[2 * b];
+        fp.u2_inf = g.inst_fs[2 * b + 1];
+    }
     const double pi = fp.pi;
     const double u1_inf_rot = fp.u1_inf * f.t1[0] + fp.u2_inf * f.t1[1] + fp.u3_inf * f.t1[2];
     const double u2_inf_rot = fp.u1_inf * f.t2[0] + fp.u2_inf * f.t2[1] + fp.u3_inf * f.t2[2];
@@ -590,7 +616,8 @@ __device__ __forceinline__ MaxPair max_pair(MaxPair a, MaxPair b) {
 
 __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, FlowParams fp,
                                                          double* partial, int* partial_pos,
-                                                         int* partial_pt, double* partial_max) {
+                                                         int* partial_pt, double* partial_max,
+                                                         double* ressq) {
     __shared__ double ssum[kBlock];
     __shared__ double smax[kBlock];
     __shared__ int spos[kBlock];
@@ -621,7 +648,7 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
         } else if (st == 2) {
             const Frame<double> f = load_frame(g.frame, k);
             double U[5], prim[5];
-            conserved_vector_Ubar(s, fp, k, f, U);
+            conserved_vector_Ubar(g, s, fp, k, f, U);
             const double dt = s.delt[k];
 #pragma unroll
             for (int r = 0; r < 5; ++r) U[r] = U[r] - dt * s.flux_res[5 * (long long)k + r];
@@ -630,6 +657,7 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
             for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = prim[r];
         }
     }
+    if (ressq && k < g.n) ressq[k] = res_sqr;
     ssum[threadIdx.x] = res_sqr;
     smax[threadIdx.x] = mp.v;
     spos[threadIdx.x] = mp.pos;
@@ -687,7 +715,8 @@ __global__ void k_supersonic(DevGeom g, DevState s, const int* out_idx, int n_ou
 __global__ void __launch_bounds__(1024) k_residual_final(const double* partial, const int* partial_pos,
                                                          const int* partial_pt, const double* partial_max,
                                                          int n_partial, int n_points,
-                                                         ResidualRecord* rec, double* hist, int hist_index) {
+                                                         ResidualRecord* rec, double* hist, int hist_index,
+                                                         const int* hist_counter) {
     __shared__ double ssum[1024];
     __shared__ double smax[1024];
     __shared__ int spos[1024];
@@ -725,15 +754,38 @@ __global__ void __launch_bounds__(1024) k_residual_final(const double* partial, 
         // the reference keeps the previous max_res_point when no residual is
         // positive; 0 here means "unchanged"
         rec->max_res_point = smax[0] > 0.0 ? spt[0] : 0;
-        if (hist) hist[hist_index] = rec->res_new;
+        if (hist) hist[hist_counter ? *hist_counter : hist_index] = rec->res_new;
     }
+}
+
+// residual of each cloud of a batch (state_update.f90:39-44,115 over the
+// cloud's own points): block b sums its cloud's per-point terms in a fixed
+// order; the history row is the iteration counter's
+__global__ void __launch_bounds__(kBlock) k_instance_residual(const double* ressq, const int* beg, const int* end,
+                                                             const double* norm, int n_inst, double* inst_hist,
+                                                             const int* counter) {
+    __shared__ double ssum[kBlock];
+    const int b = blockIdx.x;
+    double acc = 0.0;
+    for (int k = beg[b] + threadIdx.x; k < end[b]; k += kBlock) acc = acc + ressq[k];
+    ssum[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) inst_hist[(long long)(*counter) * n_inst + b] = __builtin_sqrt(ssum[0]) / norm[b];
+}
+
+__global__ void k_bump(int* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = *counter + 1;
 }
 
 }  // namespace
 
 // implemented in kernels_flux_fused.hip
 void launch_flux_fused(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
-                       hipStream_t st, int parts);
+                       hipStream_t st, int parts, bool mixed);
 
 // ---------------------------------------------------------------------------
 // launch wrappers
@@ -751,6 +803,14 @@ void launch_gather(double* dst, const double* src, const int* perm, int n, int m
     hipLaunchKernelGGL(k_gather, dim3(grid_for((long long)n * m, kBlock)), dim3(kBlock), 0, st, dst, src, perm,
                        n, m);
 }
+void launch_to_float(float* dst, const double* src, long long n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, dst, src, n);
+}
+void launch_to_double(double* dst, const float* src, long long n, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_double, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, dst, src, n);
+}
 void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_scatter, dim3(grid_for((long long)n * m, kBlock)), dim3(kBlock), 0, st, dst, src,
@@ -758,25 +818,42 @@ void launch_scatter(double* dst, const double* src, const int* perm, int n, int 
 }
 void launch_conserved_vector(const DevGeom& g, const DevState& s, hipStream_t st) {
     if (g.n <= 0) return;
-    hipLaunchKernelGGL((k_conserved_q<true, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
+    hipLaunchKernelGGL((k_conserved_q<true, false, double>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g,
+                       s);
 }
-void launch_q_variables(const DevGeom& g, const DevState& s, hipStream_t st) {
+void launch_q_variables(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
     if (g.n <= 0) return;
-    hipLaunchKernelGGL((k_conserved_q<false, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
+    if (p.mixed)
+        hipLaunchKernelGGL((k_conserved_q<false, true, float>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
+                           g, s);
+    else
+        hipLaunchKernelGGL((k_conserved_q<false, true, double>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
+                           g, s);
 }
-void launch_conserved_and_q(const DevGeom& g, const DevState& s, hipStream_t st) {
+void launch_conserved_and_q(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
     if (g.n <= 0) return;
-    hipLaunchKernelGGL((k_conserved_q<true, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
+    if (p.mixed)
+        hipLaunchKernelGGL((k_conserved_q<true, true, float>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
+                           g, s);
+    else
+        hipLaunchKernelGGL((k_conserved_q<true, true, double>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
+                           g, s);
 }
 void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams& p, double* temp,
                           hipStream_t st) {
     if (g.n <= 0) return;
+    const dim3 grid(grid_for(g.n, kBlock));
+    if (p.mixed) {
+        if (p.power == 0.0)
+            hipLaunchKernelGGL((k_qderiv_timestep<true, false, true, float>), grid, dim3(kBlock), 0, st, g, s, p);
+        else
+            hipLaunchKernelGGL((k_qderiv_timestep<true, false, false, float>), grid, dim3(kBlock), 0, st, g, s, p);
+        return;  // second-order inner iterations are FP64 only
+    }
     if (p.power == 0.0)
-        hipLaunchKernelGGL((k_qderiv_timestep<true, false, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
-                           st, g, s, p);
+        hipLaunchKernelGGL((k_qderiv_timestep<true, false, true, double>), grid, dim3(kBlock), 0, st, g, s, p);
     else
-        hipLaunchKernelGGL((k_qderiv_timestep<true, false, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
-                           st, g, s, p);
+        hipLaunchKernelGGL((k_qderiv_timestep<true, false, false, double>), grid, dim3(kBlock), 0, st, g, s, p);
     for (int it = 0; it < p.inner_iterations; ++it) {
         if (p.power == 0.0)
             hipLaunchKernelGGL(k_qderiv_inner<true>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s, p,
@@ -791,43 +868,50 @@ void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams&
 void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
     if (g.n <= 0) return;
     // no least-squares weights in the time step
-    hipLaunchKernelGGL((k_qderiv_timestep<false, true, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st,
-                       g, s, p);
+    hipLaunchKernelGGL((k_qderiv_timestep<false, true, true, double>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
+                       st, g, s, p);
 }
 void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
     if (g.n <= 0) return;
+    const dim3 grid(grid_for(g.n, kBlock));
+    if (p.mixed) {
+        if (p.power == 0.0)
+            hipLaunchKernelGGL((k_qderiv_timestep<true, true, true, float>), grid, dim3(kBlock), 0, st, g, s, p);
+        else
+            hipLaunchKernelGGL((k_qderiv_timestep<true, true, false, float>), grid, dim3(kBlock), 0, st, g, s, p);
+    } else {
+        if (p.power == 0.0)
+            hipLaunchKernelGGL((k_qderiv_timestep<true, true, true, double>), grid, dim3(kBlock), 0, st, g, s, p);
+        else
+            hipLaunchKernelGGL((k_qderiv_timestep<true, true, false, double>), grid, dim3(kBlock), 0, st, g, s, p);
+    }
+}
+template <typename T>
+void launch_lists(const DevGeom& g, const DevState& s, const FlowParams& p, const int* idx, int count,
+                  hipStream_t st) {
+    if (count <= 0) return;
     if (p.power == 0.0)
-        hipLaunchKernelGGL((k_qderiv_timestep<true, true, true>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
-                           st, g, s, p);
+        hipLaunchKernelGGL((k_flux_residual_lists<true, T>), dim3(grid_for(count, kBlock)), dim3(kBlock), 0, st, g,
+                           s, p, idx, count);
     else
-        hipLaunchKernelGGL((k_qderiv_timestep<true, true, false>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
-                           st, g, s, p);
+        hipLaunchKernelGGL((k_flux_residual_lists<false, T>), dim3(grid_for(count, kBlock)), dim3(kBlock), 0, st, g,
+                           s, p, idx, count);
 }
 void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                           hipStream_t st, int parts) {
-    launch_flux_fused(g, s, p, tl, st, parts);
-    if ((parts & 4) && tl.n_generic > 0) {
-        if (p.power == 0.0)
-            hipLaunchKernelGGL(k_flux_residual_lists<true>, dim3(grid_for(tl.n_generic, kBlock)), dim3(kBlock), 0,
-                               st, g, s, p, tl.generic, tl.n_generic);
+    launch_flux_fused(g, s, p, tl, st, parts, p.mixed != 0);
+    if (parts & 4) {
+        if (p.mixed)
+            launch_lists<float>(g, s, p, tl.generic, tl.n_generic, st);
         else
-            hipLaunchKernelGGL(k_flux_residual_lists<false>, dim3(grid_for(tl.n_generic, kBlock)), dim3(kBlock), 0,
-                               st, g, s, p, tl.generic, tl.n_generic);
+            launch_lists<double>(g, s, p, tl.generic, tl.n_generic, st);
     }
 }
 void launch_flux_residual_reference(const DevGeom& g, const DevState& s, const FlowParams& p,
                                     const TypeLists& tl, hipStream_t st) {
     const int* lists[3] = {tl.wall, tl.outer, tl.interior};
     const int counts[3] = {tl.n_wall, tl.n_outer, tl.n_interior};
-    for (int c = 0; c < 3; ++c)
-        if (counts[c] > 0) {
-            if (p.power == 0.0)
-                hipLaunchKernelGGL(k_flux_residual_lists<true>, dim3(grid_for(counts[c], kBlock)), dim3(kBlock), 0,
-                                   st, g, s, p, lists[c], counts[c]);
-            else
-                hipLaunchKernelGGL(k_flux_residual_lists<false>, dim3(grid_for(counts[c], kBlock)), dim3(kBlock),
-                                   0, st, g, s, p, lists[c], counts[c]);
-        }
+    for (int c = 0; c < 3; ++c) launch_lists<double>(g, s, p, lists[c], counts[c], st);
 }
 void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st) {
     if (g.n <= 0) return;
@@ -874,18 +958,23 @@ void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams&
 int state_update_blocks(int n) { return grid_for(n, kBlock); }
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                          double* partial, int* partial_pos, int n_partial, ResidualRecord* rec, double* hist,
-                         int hist_index, hipStream_t st) {
+                         int hist_index, hipStream_t st, const InstanceResidual* ir) {
     // partial buffer layout: [sum | max] doubles, [pos | point] ints
     double* partial_max = partial + n_partial;
     int* partial_pt = partial_pos + n_partial;
     hipLaunchKernelGGL(k_state_update, dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial, partial_pos,
-                       partial_pt, partial_max);
+                       partial_pt, partial_max, ir ? ir->ressq : nullptr);
     if (tl.n_sup_in + tl.n_sup_out > 0)
         hipLaunchKernelGGL(k_supersonic, dim3(1), dim3(1), 0, st, g, s, tl.sup_out, tl.n_sup_out, tl.sup_in,
                            tl.n_sup_in);
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, st, (const double*)partial,
                        (const int*)partial_pos, (const int*)partial_pt, (const double*)partial_max, n_partial,
-                       g.n, rec, hist, hist_index);
+                       g.n, rec, hist, hist_index, ir ? (const int*)ir->counter : (const int*)nullptr);
+    if (ir) {
+        hipLaunchKernelGGL(k_instance_residual, dim3(ir->n_inst), dim3(kBlock), 0, st, (const double*)ir->ressq,
+                           ir->beg, ir->end, ir->norm, ir->n_inst, ir->inst_hist, (const int*)ir->counter);
+        hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, st, ir->counter);
+    }
 }
 
 }  // namespace mfcfd

@@ -29,7 +29,9 @@
 // dG, times 2 at wall points) is a fixed-order tree reduction.  Each entry's
 // contribution is (Gnew - Gold) * w (c0*dels + c1*delt + c2*deln) with the
 // Cramer cofactors c of the list (k_stencil_coef), which equals the
-// reference's Cramer solve on the summed products up to rounding.
+// reference's Cramer solve on the summed products up to rounding.  T is the
+// arithmetic of the split fluxes (float in the mixed form); the weights, the
+// sums, D_inv and the new delU stay FP64.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -77,7 +79,7 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-template <bool FORWARD, bool UW>
+template <bool FORWARD, bool UW, typename T>
 __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState s, FlowParams fp,
                                                              SweepAsync a) {
     __shared__ unsigned sh_ticket;
@@ -95,6 +97,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         const int cls = g.status[i];
         const int alias_i = g.alias[i];
         const Frame<double> f = load_frame(g.frame, i);
+        const Frame<T> ft = frame_as<T>(f);
         const double4 pi4 = g.pos[i];
         int cnt[6];
         long long off[6];
@@ -115,7 +118,8 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
             const int e = base + tid;
             bool take = false;
             int k = 0, d = 0;
-            double wgt = 0.0, Gold[5], Uk[5];
+            double wgt = 0.0, Uk[5];
+            T Gold[5];
             Frame<double> fk;
             if (e < total) {
                 int rem = e;
@@ -134,10 +138,10 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                     const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
                     const double* c = g.coef + 18 * (long long)i + 3 * d;
                     wgt = wls_weight<UW>(dels, delt, deln, fp.power) * (c[0] * dels + c[1] * delt + c[2] * deln);
-                    double prim[5];
+                    T prim[5];
 #pragma unroll
-                    for (int r = 0; r < 5; ++r) prim[r] = s.prim[5 * (long long)k + r];
-                    flux_eval(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, f, prim, fp.pi, Gold);
+                    for (int r = 0; r < 5; ++r) prim[r] = T(s.prim[5 * (long long)k + r]);
+                    flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim, T(fp.pi), Gold);
 #pragma unroll
                     for (int r = 0; r < 5; ++r) Uk[r] = s.U[5 * (long long)k + r];
                     fk = load_frame(g.frame, k);
@@ -149,13 +153,16 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 __syncthreads();
             }
             if (take) {
-                double U[5], prim[5], Gnew[5];
+                double U[5], prim[5];
+                T pt[5], Gnew[5];
 #pragma unroll
                 for (int r = 0; r < 5; ++r) U[r] = dU[5 * (long long)k + r] + Uk[r];
                 conserved_to_prim(U, fk, prim);
-                flux_eval(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, f, prim, fp.pi, Gnew);
 #pragma unroll
-                for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (Gnew[r] - Gold[r]) * wgt;
+                for (int r = 0; r < 5; ++r) pt[r] = T(prim[r]);
+                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
+#pragma unroll
+                for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }
         }
         if (total == 0) {
@@ -207,12 +214,12 @@ __global__ void k_copy_sweep(double* dst, const double* src, long long n) {
     if (i < n) dst[i] = src[i];
 }
 
-template <bool FORWARD, bool UW>
+template <bool FORWARD, bool UW, typename T>
 int resident_grid(int count) {
     int dev = 0, per_cu = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sweep_async<FORWARD, UW>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sweep_async<FORWARD, UW, T>,
                                                      kSweepBlock, 0) != hipSuccess)
         return 0;
     if (per_cu < 1) per_cu = 1;
@@ -220,24 +227,41 @@ int resident_grid(int count) {
     return (int)(grid < count ? grid : count);
 }
 
+template <bool UW, typename T>
+int pair_grid(int count) {
+    const int a = resident_grid<true, UW, T>(count);
+    const int b = resident_grid<false, UW, T>(count);
+    return a < b ? a : b;
+}
+
+template <bool UW, typename T>
+void launch_pair(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
+                 const SweepAsync& bwd, int grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_sweep_async<true, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, fwd);
+    hipLaunchKernelGGL((k_sweep_async<false, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, bwd);
+}
+
 }  // namespace
 
-int sweep_async_grid(int count, bool unit_weight) {
+int sweep_async_grid(int count, bool unit_weight, bool mixed) {
     if (count <= 0) return 0;
-    const int a = unit_weight ? resident_grid<true, true>(count) : resident_grid<true, false>(count);
-    const int b = unit_weight ? resident_grid<false, true>(count) : resident_grid<false, false>(count);
-    return a < b ? a : b;
+    if (mixed) return unit_weight ? pair_grid<true, float>(count) : pair_grid<false, float>(count);
+    return unit_weight ? pair_grid<true, double>(count) : pair_grid<false, double>(count);
 }
 
 void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
                          const SweepAsync& bwd, int grid, hipStream_t st) {
     if (fwd.count > 0 && grid > 0) {
-        if (p.power == 0.0) {
-            hipLaunchKernelGGL((k_sweep_async<true, true>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, fwd);
-            hipLaunchKernelGGL((k_sweep_async<false, true>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, bwd);
+        if (p.mixed) {
+            if (p.power == 0.0)
+                launch_pair<true, float>(g, s, p, fwd, bwd, grid, st);
+            else
+                launch_pair<false, float>(g, s, p, fwd, bwd, grid, st);
         } else {
-            hipLaunchKernelGGL((k_sweep_async<true, false>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, fwd);
-            hipLaunchKernelGGL((k_sweep_async<false, false>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, bwd);
+            if (p.power == 0.0)
+                launch_pair<true, double>(g, s, p, fwd, bwd, grid, st);
+            else
+                launch_pair<false, double>(g, s, p, fwd, bwd, grid, st);
         }
     }
     // implicit_forward_and_reverse_sweeps.f90:86-88

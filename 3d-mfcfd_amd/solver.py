@@ -61,13 +61,16 @@ class Residual(C.Structure):
 _lib = None
 
 EXPORTS = ["mfcfd_create", "mfcfd_destroy", "mfcfd_last_error", "mfcfd_abi_version",
-           "mfcfd_set_params", "mfcfd_bind_geometry", "mfcfd_generate_split_stencils",
+           "mfcfd_set_params", "mfcfd_bind_geometry", "mfcfd_bind_geometry_arrays",
+           "mfcfd_generate_split_stencils",
            "mfcfd_aliasing", "mfcfd_upload", "mfcfd_download",
            "mfcfd_compute_conserved_vector", "mfcfd_eval_q_variables",
            "mfcfd_eval_q_derivatives", "mfcfd_timestep_delt", "mfcfd_eval_flux_residual",
            "mfcfd_implicit_forward_backward_sweeps", "mfcfd_state_update", "mfcfd_fpi_solver",
            "mfcfd_run", "mfcfd_get_residual", "mfcfd_synchronize", "mfcfd_sweep_info",
-           "mfcfd_time_phases", "mfcfd_point_classes"]
+           "mfcfd_time_phases", "mfcfd_point_classes", "mfcfd_point_class_map",
+           "mfcfd_kernel_timing", "mfcfd_kernel_time", "mfcfd_bind_batch", "mfcfd_batch_info",
+           "mfcfd_run_graph"]
 PHASES = ["conserved_q", "qderiv_timestep", "flux_interior", "flux_boundary", "sweeps",
           "state_update"]
 
@@ -101,6 +104,12 @@ def lib():
         L.mfcfd_sweep_info.argtypes = [vp, _ip, _ip, _ip]
         L.mfcfd_time_phases.argtypes = [vp, C.c_int, C.c_int, _dp]
         L.mfcfd_point_classes.argtypes = [vp, _ip, _ip, _ip]
+        L.mfcfd_point_class_map.argtypes = [vp, _ip]
+        L.mfcfd_kernel_timing.argtypes = [vp, C.c_int]
+        L.mfcfd_kernel_time.argtypes = [vp, _dp, C.POINTER(C.c_longlong)]
+        L.mfcfd_bind_batch.argtypes = [vp, C.c_int, C.POINTER(Geometry), _dp, _dp]
+        L.mfcfd_batch_info.argtypes = [vp, _ip, _ip]
+        L.mfcfd_run_graph.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -154,6 +163,27 @@ def _cloud_arrays(cloud):
             "status": i(cloud.status), "nbhs": i(cloud.nbhs), "conn": i(cloud.conn)}
 
 
+def _geometry(cloud, split=None, alias=None):
+    """mfcfd_geometry of one cloud and the arrays it points into."""
+    a = _cloud_arrays(cloud)
+    cnt, lists = split if split is not None else split_stencils(cloud)
+    al, pwa = alias if alias is not None else alias_order(cloud)
+    keep = [a, np.ascontiguousarray(cnt, dtype=np.int32), np.ascontiguousarray(lists, dtype=np.int32),
+            np.ascontiguousarray(al, dtype=np.int32), np.ascontiguousarray(pwa, dtype=np.int32)]
+    _, cnt, lists, al, pwa = keep
+    g = Geometry()
+    g.n = cloud.n
+    for nm in ("x", "y", "z", "tan1", "tan2", "nor", "min_dist"):
+        setattr(g, nm, _d(a[nm]))
+    for nm in ("status", "nbhs", "conn"):
+        setattr(g, nm, _i(a[nm]))
+    g.split_nbhs = (_ip * 6)(*[_i(cnt[d]) for d in range(6)])
+    g.split_conn = (_ip * 6)(*[_i(lists[d]) for d in range(6)])
+    g.alias = _i(al)
+    g.point_with_alias = _i(pwa)
+    return g, keep
+
+
 class Solver:
     """One device context for one point cloud.
 
@@ -165,9 +195,16 @@ class Solver:
     def __init__(self, cloud, mach, aoa, cfl, mode="explicit", precision="fp64",
                  vl_const=2.0, power=0.0, inner_iterations=0, max_iters=1000, device=0,
                  split=None, alias=None, flux_order="fused", sweep_schedule="async"):
-        L = lib()
-        self.n = cloud.n
+        self._create(cloud.n, mach, aoa, cfl, mode, precision, vl_const, power, inner_iterations,
+                     max_iters, device, flux_order, sweep_schedule)
         self.cloud = cloud
+        g, self._keep = _geometry(cloud, split, alias)
+        _check(lib().mfcfd_bind_geometry(self._ctx, C.byref(g)))
+
+    def _create(self, n, mach, aoa, cfl, mode, precision, vl_const, power, inner_iterations,
+                max_iters, device, flux_order, sweep_schedule):
+        L = lib()
+        self.n = n
         self._ctx = C.c_void_p()
         _check(L.mfcfd_create(device, C.byref(self._ctx)))
         self.params = Params(mach, aoa, cfl, vl_const, power, inner_iterations,
@@ -176,26 +213,6 @@ class Solver:
                              1 if flux_order == "lists" else 0,
                              1 if sweep_schedule == "levels" else 0)
         _check(L.mfcfd_set_params(self._ctx, C.byref(self.params)))
-        self._arrs = _cloud_arrays(cloud)
-        cnt, lists = split if split is not None else split_stencils(cloud)
-        al, pwa = alias if alias is not None else alias_order(cloud)
-        self._keep = [np.ascontiguousarray(cnt, dtype=np.int32),
-                      np.ascontiguousarray(lists, dtype=np.int32),
-                      np.ascontiguousarray(al, dtype=np.int32),
-                      np.ascontiguousarray(pwa, dtype=np.int32)]
-        cnt, lists, al, pwa = self._keep
-        a = self._arrs
-        g = Geometry()
-        g.n = self.n
-        for nm in ("x", "y", "z", "tan1", "tan2", "nor", "min_dist"):
-            setattr(g, nm, _d(a[nm]))
-        for nm in ("status", "nbhs", "conn"):
-            setattr(g, nm, _i(a[nm]))
-        g.split_nbhs = (_ip * 6)(*[_i(cnt[d]) for d in range(6)])
-        g.split_conn = (_ip * 6)(*[_i(lists[d]) for d in range(6)])
-        g.alias = _i(al)
-        g.point_with_alias = _i(pwa)
-        _check(L.mfcfd_bind_geometry(self._ctx, C.byref(g)))
 
     def close(self):
         if self._ctx:
@@ -272,7 +289,69 @@ class Solver:
         _check(lib().mfcfd_point_classes(self._ctx, C.byref(a), C.byref(b), C.byref(c)))
         return {"fused_interior": a.value, "fused_wall": b.value, "listwise": c.value}
 
+    def point_class_map(self):
+        """Per host point: 0 fused interior, 1 fused wall, 2 list-by-list."""
+        out = np.zeros(self.n, dtype=np.int32)
+        _check(lib().mfcfd_point_class_map(self._ctx, out.ctypes.data_as(_ip)))
+        return out
+
+    def kernel_timing(self, enable=True):
+        """Record HIP events around every fused interior flux launch."""
+        _check(lib().mfcfd_kernel_timing(self._ctx, 1 if enable else 0))
+
+    def kernel_time(self):
+        """(summed ms, launches) of the fused interior flux kernel since
+        kernel_timing() was called."""
+        ms, k = C.c_double(), C.c_longlong()
+        _check(lib().mfcfd_kernel_time(self._ctx, C.byref(ms), C.byref(k)))
+        return ms.value, k.value
+
+    def run_graph(self, t0, n_iters):
+        """run() with the iteration captured as a HIP graph; returns the
+        res_new history and the per-cloud residual history (n_iters, clouds)."""
+        rn = np.zeros(n_iters)
+        ni = self.batch_info()[0]
+        inst = np.zeros((n_iters, ni))
+        _check(lib().mfcfd_run_graph(self._ctx, t0, n_iters, _d(rn), _d(inst)))
+        return rn, inst
+
+    def batch_info(self):
+        k = C.c_int()
+        _check(lib().mfcfd_batch_info(self._ctx, C.byref(k), None))
+        counts = np.zeros(k.value, dtype=np.int32)
+        _check(lib().mfcfd_batch_info(self._ctx, C.byref(k), counts.ctypes.data_as(_ip)))
+        return k.value, counts
+
     def sweep_info(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _check(lib().mfcfd_sweep_info(self._ctx, C.byref(a), C.byref(b), C.byref(c)))
         return {"levels_fwd": a.value, "levels_bwd": b.value, "swept": c.value}
+
+
+class BatchSolver(Solver):
+    """Several independent clouds in one context (mfcfd_bind_batch): every
+    launch covers all of them; each keeps its own Mach and angle of attack.
+    Arrays of upload/download hold the clouds one after the other."""
+
+    def __init__(self, clouds, machs, aoas, cfl, mode="explicit", precision="fp64", vl_const=2.0,
+                 power=0.0, max_iters=1000, device=0, flux_order="fused", sweep_schedule="async"):
+        n = int(sum(c.n for c in clouds))
+        self._create(n, float(machs[0]), float(aoas[0]), cfl, mode, precision, vl_const, power, 0,
+                     max_iters, device, flux_order, sweep_schedule)
+        self.clouds = list(clouds)
+        self.offsets = np.concatenate([[0], np.cumsum([c.n for c in clouds])]).astype(np.int64)
+        geoms = (Geometry * len(clouds))()
+        self._keep = []
+        for b, cl in enumerate(clouds):
+            g, keep = _geometry(cl)
+            geoms[b] = g
+            self._keep.append(keep)
+        self._geoms = geoms
+        m = np.ascontiguousarray(machs, dtype=np.float64)
+        a = np.ascontiguousarray(aoas, dtype=np.float64)
+        _check(lib().mfcfd_bind_batch(self._ctx, len(clouds), geoms, _d(m), _d(a)))
+
+    def split(self, arr):
+        """Per-cloud views of a concatenated (N, ...) array."""
+        return [arr[self.offsets[b]:self.offsets[b + 1]] for b in range(len(self.clouds))]
+

@@ -10,14 +10,21 @@ W warmup iterations, then K timed iterations, each side bracketed by a
 device synchronize.
 
 Reported beside the throughput:
-  roofline      the interior flux-residual kernel (the dominant kernel):
-                algorithmic bytes per launch / mean launch time measured with
-                HIP events on the library's stream, against 8 TB/s;
+  roofline      the fused interior flux-residual kernel (the dominant
+                kernel): algorithmic bytes per launch (the reference's
+                eval_flux_residual traffic, 388 + 4*(6 + list entries) bytes
+                per point, for the points that launch processes) / its mean
+                launch time, from HIP events recorded on the library's stream
+                around every launch of the timed iterations, against 8 TB/s;
                 traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE and
                 WRITE_SIZE when a summary exists in profiles/;
   cpu_baseline  the C restatement of the reference (bit-identical to the
                 reference Fortran on the golden fixtures), one core, one
                 iteration of the same cloud from the GPU's state.
+  mixed         the same W + K iterations in the mixed-precision form (FP32
+                q-variables, gradients and split fluxes): its rate, its
+                speed-up over FP64 and its wall-Cp error against the FP64
+                state after the same iterations.
 """
 
 from __future__ import annotations
@@ -37,6 +44,7 @@ PKG = importlib.import_module("3d-mfcfd_amd")
 pc = PKG.pointcloud
 solver = PKG.solver
 roofline = importlib.import_module("3d-mfcfd_amd.roofline")
+postprocess = PKG.postprocess
 
 METRIC = "Mpoint-iter/s on 1 GPU, FP64 vs mixed, %HBM roofline, wall-Cp err vs FP64"
 
@@ -76,6 +84,19 @@ def cpu_baseline(cloud, cfg, state, t):
                       f"C restatement of the reference on one host core, {dt:.1f} s"}
 
 
+def copy_probe(mib):
+    """A plain device copy of known size (reads and writes mib MiB each
+    launch, three launches) for scaling the HBM traffic counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 launches of {mib} MiB read + {mib} MiB written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,8 +104,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c2_2m", choices=sorted(pc.CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--phase-iters", type=int, default=3)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs "
               f"the q-LSKUM iteration on a single MI355X)", file=sys.stderr)
@@ -108,10 +135,13 @@ def main():
         s.run(1, W)
     s.synchronize()
     log(f"warmup {W} done")
+    s.kernel_timing(True)
     t0 = time.perf_counter()
     rn, _ = s.run(W + 1, K)
     s.synchronize()
     elapsed = time.perf_counter() - t0
+    flux_total_ms, flux_launches = s.kernel_time()
+    s.kernel_timing(False)
     ms_per_step = elapsed / K * 1000.0
     value = cloud.n * K / elapsed / 1e6
     log(f"timed {K} iterations: {ms_per_step:.3f} ms/iter, {value:.2f} Mpoint-iter/s, "
@@ -121,14 +151,45 @@ def main():
     log("phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
     t_next = W + K + args.phase_iters + 1
     by = roofline.iteration_bytes(cloud.status, cnt, cloud.nbhs, cfg["mode"])
-    flux_ms = phases["flux_interior"]
-    achieved = by["flux_interior"] / (flux_ms * 1e-3) / 1e9
+    cls = s.point_class_map()
+    flux_bytes = int(roofline.flux_bytes_per_point(cloud.status, cnt, cloud.nbhs)[cls == 0].sum())
+    flux_ms = flux_total_ms / max(flux_launches, 1)
+    log(f"fused interior flux kernel: {flux_launches} launches, mean {flux_ms:.4f} ms, "
+        f"{int(np.sum(cls == 0))} points, {flux_bytes} algorithmic bytes per launch")
+    achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 5),
             "traffic": read_traffic(args.config),
-            "kernel": "k_flux_residual<interior>",
-            "bytes_per_launch": by["flux_interior"], "launch_ms": round(flux_ms, 4)}
+            "kernel": "k_flux_fused<0,true> (interior points on the fused path)",
+            "bytes_per_launch": flux_bytes, "launch_ms": round(flux_ms, 4),
+            "launches_timed": flux_launches}
     step_gbs = by["total"] / (ms_per_step * 1e-3) / 1e9
+
+    prim_fp64 = s.download("prim")
+    mixed = None
+    if not args.no_mixed:
+        # same start, same W + K iterations, mixed precision
+        m = solver.Solver(cloud, cfg["mach"], cfg["aoa"], cfg["cfl"], mode=cfg["mode"],
+                          precision="mixed", split=(cnt, lists), alias=alias)
+        m.upload("prim", prim0)
+        if W > 0:
+            m.run(1, W)
+        m.synchronize()
+        t0 = time.perf_counter()
+        rn_m, _ = m.run(W + 1, K)
+        m.synchronize()
+        el_m = time.perf_counter() - t0
+        prim_m = m.download("prim")
+        m.close()
+        wall = cloud.status == pc.STATUS_WALL
+        cp64 = postprocess.wall_cp(prim_fp64[wall], cfg["mach"])
+        cpm = postprocess.wall_cp(prim_m[wall], cfg["mach"])
+        v_m = cloud.n * K / el_m / 1e6
+        mixed = {"value": round(v_m, 3), "ms_per_step": round(el_m / K * 1000.0, 4),
+                 "speedup_vs_fp64": round(v_m / value, 3),
+                 "wall_cp_err_vs_fp64": float(np.max(np.abs(cpm - cp64)) / np.max(np.abs(cp64))),
+                 "res_new_last": float(rn_m[-1]), "res_new_last_fp64": float(rn[-1])}
+        log(f"mixed: {mixed}")
 
     cpu = None
     if not args.no_cpu:
@@ -151,6 +212,7 @@ def main():
                    "points": cloud.n, "mode": cfg["mode"], "precision": "fp64"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "mixed": mixed,
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
         "step_algorithmic_gbs": round(step_gbs, 2),
         "step_hbm_frac": round(step_gbs / roofline.HBM_PEAK_GBS, 5),

@@ -38,6 +38,11 @@ extern "C" {
 #define MFCFD_EXPLICIT 0       /* U <- U - delt*R (state_update.f90:35,67) */
 #define MFCFD_IMPLICIT 1       /* LU-SGS sweeps (implicit_forward_and_reverse_sweeps.f90) */
 #define MFCFD_FP64 0
+/* MIXED: q, qm, dq, the q-derivative right-hand sides and the split fluxes
+ * (flux residual and LU-SGS sweeps) in FP32; positions, least-squares
+ * geometry and 3x3 solves, Cramer weights, the flux sums, D_inv, the state
+ * update and the residual in FP64.  Fused flux order and sync-free sweeps
+ * only, no second-order inner iterations. */
 #define MFCFD_MIXED 1
 
 /* array ids for upload/download; shapes are the reference's */
@@ -151,6 +156,18 @@ int mfcfd_generate_split_stencils(int n, const double *x, const double *y, const
                                   int *split_nbhs[6], int *split_conn[6]);
 int mfcfd_aliasing(int n, const int *nbhs, const int *conn, int *alias, int *point_with_alias);
 
+/* a batch of independent clouds iterated together (no counterpart in the
+ * reference, which holds one cloud): the clouds are laid out one after the
+ * other in device memory, every kernel launch covers all of them, and each
+ * keeps its own Mach and angle of attack (the far-field state of
+ * conserved_vector_Ubar.f90) and its own residual.  Host arrays of
+ * upload/download then hold the clouds' points one cloud after the other.
+ * The other parameters (CFL, mode, precision, ...) are shared. */
+int mfcfd_bind_batch(mfcfd_ctx *ctx, int n_instances, const mfcfd_geometry *geoms,
+                     const double *mach, const double *aoa);
+/* number of clouds bound (1 for mfcfd_bind_geometry) and their point counts */
+int mfcfd_batch_info(mfcfd_ctx *ctx, int *n_instances, int *counts);
+
 /* state transfer by array id (MFCFD_ARR_*), reference layout */
 int mfcfd_upload(mfcfd_ctx *ctx, int array_id, const double *host);
 int mfcfd_download(mfcfd_ctx *ctx, int array_id, double *host);
@@ -172,6 +189,13 @@ int mfcfd_fpi_solver(mfcfd_ctx *ctx, int t, double *res_new, double *residue);
 /* q_lskum.f90:28-32 without the printing: iterations t0..t0+n-1 with no host
  * round trip; per-iteration res_new and residue (either may be NULL) */
 int mfcfd_run(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, double *residue_hist);
+
+/* mfcfd_run with the iteration captured once as a HIP graph and replayed
+ * n_iters times; res_new_hist as in mfcfd_run (the whole bound set), and
+ * inst_res_hist[it*n_instances + b] the residual of cloud b alone
+ * (sqrt(sum over its wall and interior points)/its point count).  Either
+ * may be NULL. */
+int mfcfd_run_graph(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, double *inst_res_hist);
 
 int mfcfd_get_residual(mfcfd_ctx *ctx, mfcfd_residual *out);
 int mfcfd_synchronize(mfcfd_ctx *ctx);
@@ -197,6 +221,17 @@ int mfcfd_sweep_info(mfcfd_ctx *ctx, int *levels_fwd, int *levels_bwd, int *n_sw
  * and how many the list-by-list kernel (outer points and points whose lists
  * were altered by the 22-column spill) */
 int mfcfd_point_classes(mfcfd_ctx *ctx, int *n_fused_interior, int *n_fused_wall, int *n_listwise);
+
+/* diagnostics: the same classes per point, host order (N ints):
+ * 0 fused interior, 1 fused wall, 2 list-by-list */
+int mfcfd_point_class_map(mfcfd_ctx *ctx, int *cls);
+
+/* diagnostics: with timing enabled, every iteration records HIP events on
+ * the context's stream around the fused interior flux launch (the dominant
+ * kernel); mfcfd_kernel_time returns the summed milliseconds and the number
+ * of timed launches since mfcfd_kernel_timing was last called */
+int mfcfd_kernel_timing(mfcfd_ctx *ctx, int enable);
+int mfcfd_kernel_time(mfcfd_ctx *ctx, double *total_ms, long long *launches);
 
 #ifdef __cplusplus
 }

@@ -128,8 +128,8 @@ struct mfcfd_ctx {
     DevBuf<int> idx_fused_interior, idx_fused_wall, idx_generic;
     TypeLists tl{};
     // state
-    DevBuf<double> prim, U, q, qm, dq, delt, flux_res, delUp, delUn, dinv, temp, staging;
-    DevBuf<float> q32, qm32, dq32;   // mixed precision
+    DevBuf<double> prim, U, qrec, delt, flux_res, delUp, delUn, dinv, temp, staging, staging2;
+    DevBuf<float> qrec32;   // mixed precision records
     // residual
     DevBuf<double> partial;
     DevBuf<int> partial_pos;
@@ -153,6 +153,8 @@ struct mfcfd_ctx {
     DevBuf<int> inst_id, inst_beg, inst_end, hist_ctr;
     DevBuf<double> inst_fs, inst_norm, ressq, inst_hist;
     std::vector<int> inst_count;
+    // launch shapes chosen by mfcfd_autotune (defaults until then)
+    LaunchShapes shapes;
     // one iteration captured as a HIP graph (mfcfd_run_graph)
     hipGraphExec_t graph_exec = nullptr;
     bool graph_stale = true;
@@ -194,22 +196,18 @@ struct mfcfd_ctx {
         DevState s;
         s.prim = prim.p;
         s.U = U.p;
-        s.q = q.p;
-        s.qm = qm.p;
-        s.dq = dq.p;
+        s.qrec = qrec.p;
         s.delt = delt.p;
         s.flux_res = flux_res.p;
         s.delUp = delUp.p;
         s.delUn = delUn.p;
         s.dinv = dinv.p;
-        s.q32 = q32.p;
-        s.qm32 = qm32.p;
-        s.dq32 = dq32.p;
+        s.qrec32 = qrec32.p;
         return s;
     }
     void release_all() {
-        DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &q, &qm, &dq, &delt, &flux_res, &delUp, &delUn,
-                                 &dinv, &temp, &staging, &partial, &hist};
+        DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &qrec, &delt, &flux_res, &delUp, &delUn,
+                                 &dinv, &temp, &staging, &staging2, &partial, &hist};
         for (auto* b : dbl) b->release();
         DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
@@ -218,9 +216,7 @@ struct mfcfd_ctx {
         for (auto* b : ints) b->release();
         fwd_dep_ptr.release();
         bwd_dep_ptr.release();
-        q32.release();
-        qm32.release();
-        dq32.release();
+        qrec32.release();
         sweep_ctr.release();
         DevBuf<int>* ib[] = {&inst_id, &inst_beg, &inst_end, &hist_ctr};
         for (auto* b : ib) b->release();
@@ -262,13 +258,13 @@ struct ArrayDesc {
     int per_point;
 };
 
-// FP32 array the mixed form keeps for an array id (null: none)
-float* mixed_copy_of(mfcfd_ctx* c, int id) {
+// q, qm and dq live in the per-point records: field offset and width
+bool record_field(int id, int* off, int* m) {
     switch (id) {
-        case MFCFD_ARR_Q: return c->q32.p;
-        case MFCFD_ARR_QM: return c->qm32.p;
-        case MFCFD_ARR_DQ: return c->dq32.p;
-        default: return nullptr;
+        case MFCFD_ARR_Q: *off = kQOff; *m = 5; return true;
+        case MFCFD_ARR_QM: *off = kQmOff; *m = 10; return true;
+        case MFCFD_ARR_DQ: *off = kDqOff; *m = 15; return true;
+        default: return false;
     }
 }
 
@@ -276,9 +272,9 @@ ArrayDesc array_of(mfcfd_ctx* c, int id) {
     switch (id) {
         case MFCFD_ARR_PRIM: return {&c->prim, 5};
         case MFCFD_ARR_U: return {&c->U, 5};
-        case MFCFD_ARR_Q: return {&c->q, 5};
-        case MFCFD_ARR_QM: return {&c->qm, 10};
-        case MFCFD_ARR_DQ: return {&c->dq, 15};
+        case MFCFD_ARR_Q: return {&c->staging2, 5};
+        case MFCFD_ARR_QM: return {&c->staging2, 10};
+        case MFCFD_ARR_DQ: return {&c->staging2, 15};
         case MFCFD_ARR_DELT: return {&c->delt, 1};
         case MFCFD_ARR_FLUX_RES: return {&c->flux_res, 5};
         case MFCFD_ARR_DELUP: return {&c->delUp, 5};
@@ -298,6 +294,7 @@ void flux(mfcfd_ctx* c, int parts) {
         launch_flux_residual_reference(c->geom(), c->state(), c->fp, c->tl, c->stream);
         return;
     }
+    const LaunchShapes& ls = c->shapes;
     if (c->time_flux && (parts & 1)) {
         if (c->tev_used + 2 > c->tev.size()) {
             hipEvent_t a, b;
@@ -312,13 +309,13 @@ void flux(mfcfd_ctx* c, int parts) {
         }
         if (c->tev_used + 2 <= c->tev.size()) {
             hipEventRecord(c->tev[c->tev_used], c->stream);
-            launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream, 1);
+            launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream, 1, ls);
             hipEventRecord(c->tev[c->tev_used + 1], c->stream);
             c->tev_used += 2;
             parts &= ~1;
         }
     }
-    if (parts) launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream, parts);
+    if (parts) launch_flux_residual(c->geom(), c->state(), c->fp, c->tl, c->stream, parts, ls);
 }
 
 // after a stream synchronize: fold the recorded launch times into the totals
@@ -366,7 +363,8 @@ int do_sweeps(mfcfd_ctx* c) {
     bwd.seq = c->bwd_seq.p;
     bwd.dep_ptr = c->bwd_dep_ptr.p;
     bwd.dep = c->bwd_dep.p;
-    launch_sweeps_async(g, s, c->fp, fwd, bwd, c->fp.mixed ? c->sweep_grid_mixed : c->sweep_grid, c->stream);
+    const int grid = c->fp.mixed ? c->sweep_grid_mixed : c->sweep_grid;
+    launch_sweeps_async(g, s, c->fp, fwd, bwd, std::max(1, grid / std::max(1, c->shapes.sweep_div)), c->stream);
     return launch_error("implicit sweeps");
 }
 
@@ -384,7 +382,7 @@ int do_iteration(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResid
         launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
         launch_timestep(g, s, c->fp, c->stream);
     } else {
-        launch_qderiv_timestep(g, s, c->fp, c->stream);
+        launch_qderiv_timestep(g, s, c->fp, c->stream, c->shapes.qderiv);
     }
     flux(c, kFluxAll);
     int rc = launch_error("iteration");
@@ -741,23 +739,18 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     if ((rc = launch_error("stencil coefficients"))) return rc;
 
     // flow state (zero until uploaded; initial_conditions.f90:23-24 zero delUp/delUn)
-    DevBuf<double>* st5[] = {&c->prim, &c->U, &c->q, &c->flux_res, &c->delUp, &c->delUn};
+    DevBuf<double>* st5[] = {&c->prim, &c->U, &c->flux_res, &c->delUp, &c->delUn};
     for (auto* b : st5) {
         if (b->alloc(5ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "state arrays");
         HIP_TRY(hipMemsetAsync(b->p, 0, 5ULL * n * sizeof(double), st));
     }
-    if (c->qm.alloc(10ULL * n) != hipSuccess || c->dq.alloc(15ULL * n) != hipSuccess ||
+    if (c->qrec.alloc((size_t)kQRec * n) != hipSuccess || c->staging2.alloc(15ULL * n) != hipSuccess ||
         c->delt.alloc((size_t)n) != hipSuccess || c->dinv.alloc((size_t)n) != hipSuccess ||
         c->staging.alloc(15ULL * n) != hipSuccess)
         return fail(MFCFD_ENOMEM, "state arrays");
-    HIP_TRY(hipMemsetAsync(c->qm.p, 0, 10ULL * n * sizeof(double), st));
-    HIP_TRY(hipMemsetAsync(c->dq.p, 0, 15ULL * n * sizeof(double), st));
-    if (c->q32.alloc(5ULL * n) != hipSuccess || c->qm32.alloc(10ULL * n) != hipSuccess ||
-        c->dq32.alloc(15ULL * n) != hipSuccess)
-        return fail(MFCFD_ENOMEM, "FP32 q arrays");
-    HIP_TRY(hipMemsetAsync(c->q32.p, 0, 5ULL * n * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(c->qm32.p, 0, 10ULL * n * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(c->dq32.p, 0, 15ULL * n * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->qrec.p, 0, (size_t)kQRec * n * sizeof(double), st));
+    if (c->qrec32.alloc((size_t)kQRec * n) != hipSuccess) return fail(MFCFD_ENOMEM, "FP32 q records");
+    HIP_TRY(hipMemsetAsync(c->qrec32.p, 0, (size_t)kQRec * n * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->delt.p, 0, (size_t)n * sizeof(double), st));
     HIP_TRY(hipMemsetAsync(c->dinv.p, 0, (size_t)n * sizeof(double), st));
     if (c->fp.inner_iterations > 0 && c->temp.alloc(15ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "temp");
@@ -1015,9 +1008,8 @@ int mfcfd_upload(mfcfd_ctx* c, int id, const double* host) {
     const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
     HIP_TRY(hipMemcpyAsync(c->staging.p, host, bytes, hipMemcpyHostToDevice, c->stream));
     launch_gather(a.buf->p, c->staging.p, c->perm.p, c->n, a.per_point, c->stream);
-    // mixed form: the double array is staging for the FP32 copy the kernels read
-    if (float* f32 = c->fp.mixed ? mixed_copy_of(c, id) : nullptr)
-        launch_to_float(f32, a.buf->p, (long long)a.per_point * c->n, c->stream);
+    int off = 0, m = 0;
+    if (record_field(id, &off, &m)) launch_rec_put(c->state(), c->fp.mixed != 0, a.buf->p, c->n, m, off, c->stream);
     if ((rc = launch_error("upload permutation"))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MFCFD_OK;
@@ -1029,8 +1021,8 @@ int mfcfd_download(mfcfd_ctx* c, int id, double* host) {
     ArrayDesc a = array_of(c, id);
     if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
     const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
-    if (float* f32 = c->fp.mixed ? mixed_copy_of(c, id) : nullptr)
-        launch_to_double(a.buf->p, f32, (long long)a.per_point * c->n, c->stream);
+    int off = 0, m = 0;
+    if (record_field(id, &off, &m)) launch_rec_get(c->state(), c->fp.mixed != 0, a.buf->p, c->n, m, off, c->stream);
     launch_scatter(c->staging.p, a.buf->p, c->perm.p, c->n, a.per_point, c->stream);
     if ((rc = launch_error("download permutation"))) return rc;
     HIP_TRY(hipMemcpyAsync(host, c->staging.p, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -1179,6 +1171,147 @@ int mfcfd_run_graph(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, dou
     return MFCFD_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// mean milliseconds of `repeats` back-to-back launches of fn (after one
+// untimed launch)
+template <typename F>
+int time_launches(mfcfd_ctx* c, int repeats, F fn, double* ms_out) {
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(MFCFD_EHIP, "hipEventCreate");
+    }
+    fn();
+    hipEventRecord(a, c->stream);
+    for (int r = 0; r < repeats; ++r) fn();
+    hipEventRecord(b, c->stream);
+    int rc = launch_error("autotune");
+    float ms = 0.f;
+    if (!rc && hipEventSynchronize(b) != hipSuccess) rc = fail(MFCFD_EHIP, "event synchronize");
+    if (!rc) hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms_out = ms / repeats;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (repeats <= 0) repeats = 3;
+    const DevGeom g = c->geom();
+    const DevState s = c->state();
+    const int blocks[3] = {64, 128, 256};
+    mfcfd_tuning t{};
+    // inputs of the timed kernels: the first steps of an iteration from prim
+    launch_conserved_and_q(g, s, c->fp, c->stream);
+    LaunchShapes best = c->shapes;
+    if (c->fp.inner_iterations == 0) {
+        const int waves[3] = {2, 4, 6};
+        double best_ms = 1e30;
+        for (int bl : blocks)
+            for (int w : waves) {
+                KernelShape ks;
+                ks.block = bl;
+                ks.waves = w;
+                double ms = 0.0;
+                if ((rc = time_launches(c, repeats, [&] { launch_qderiv_timestep(g, s, c->fp, c->stream, ks); }, &ms)))
+                    return rc;
+                if (bl == 256 && w == 4) t.qderiv_default_ms = ms;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best.qderiv = ks;
+                }
+            }
+        t.qderiv_ms = best_ms;
+        launch_qderiv_timestep(g, s, c->fp, c->stream, best.qderiv);
+    } else {
+        launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
+        launch_timestep(g, s, c->fp, c->stream);
+    }
+    if (c->params.flux_order == MFCFD_FLUX_FUSED) {
+        const int waves[4] = {1, 2, 3, 4};
+        for (int part = 1; part <= 2; ++part) {
+            const int count = part == 1 ? c->tl.n_fused_interior : c->tl.n_fused_wall;
+            if (count == 0) continue;
+            double best_ms = 1e30;
+            KernelShape pick;
+            for (int bl : blocks)
+                for (int w : waves) {
+                    LaunchShapes ls = best;
+                    (part == 1 ? ls.flux_interior : ls.flux_wall) = KernelShape{bl, w};
+                    double ms = 0.0;
+                    if ((rc = time_launches(c, repeats,
+                                            [&] { launch_flux_residual(g, s, c->fp, c->tl, c->stream, part, ls); },
+                                            &ms)))
+                        return rc;
+                    if (bl == 256 && w == 2) (part == 1 ? t.flux_interior_default_ms : t.flux_wall_default_ms) = ms;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        pick = KernelShape{bl, w};
+                    }
+                }
+            (part == 1 ? best.flux_interior : best.flux_wall) = pick;
+            (part == 1 ? t.flux_interior_ms : t.flux_wall_ms) = best_ms;
+        }
+    }
+    flux(c, kFluxAll);
+    // the sweeps are a function of flux_res, prim, U and delt only (each
+    // point reads its dependencies' values of the same sweep), so repeating
+    // them leaves the state as one sweep would
+    if (c->params.mode == MFCFD_IMPLICIT && c->implicit_supported &&
+        c->params.sweep_schedule == MFCFD_SWEEP_ASYNC) {
+        const int divs[3] = {1, 2, 4};
+        double best_ms = 1e30;
+        for (int dv : divs) {
+            c->shapes.sweep_div = dv;
+            double ms = 0.0;
+            if ((rc = time_launches(c, repeats, [&] { (void)do_sweeps(c); }, &ms))) return rc;
+            if (dv == 1) t.sweeps_default_ms = ms;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.sweep_div = dv;
+            }
+        }
+        t.sweeps_ms = best_ms;
+        if ((rc = fetch_record(c))) return rc;  // surfaces a sweep stall
+    }
+    c->shapes = best;
+    c->graph_stale = true;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    t.flux_interior_block = best.flux_interior.block;
+    t.flux_interior_waves = best.flux_interior.waves;
+    t.flux_wall_block = best.flux_wall.block;
+    t.flux_wall_waves = best.flux_wall.waves;
+    t.qderiv_block = best.qderiv.block;
+    t.qderiv_waves = best.qderiv.waves;
+    t.sweep_div = best.sweep_div;
+    if (out) *out = t;
+    return MFCFD_OK;
+}
+
+int mfcfd_set_tuning(mfcfd_ctx* c, const mfcfd_tuning* in) {
+    if (!c || !in) return fail(MFCFD_EINVAL, "null argument");
+    auto ok_block = [](int b) { return b == 0 || b == 64 || b == 128 || b == 256; };
+    if (!ok_block(in->flux_interior_block) || !ok_block(in->flux_wall_block) || !ok_block(in->qderiv_block))
+        return fail(MFCFD_EINVAL, "block sizes are 64, 128 or 256 (0: default)");
+    if (in->sweep_div < 0 || in->sweep_div > 64) return fail(MFCFD_EINVAL, "sweep_div %d", in->sweep_div);
+    c->shapes.flux_interior = KernelShape{in->flux_interior_block, in->flux_interior_waves};
+    c->shapes.flux_wall = KernelShape{in->flux_wall_block, in->flux_wall_waves};
+    c->shapes.qderiv = KernelShape{in->qderiv_block, in->qderiv_waves};
+    c->shapes.sweep_div = in->sweep_div > 0 ? in->sweep_div : 1;
+    c->graph_stale = true;
+    return MFCFD_OK;
+}
+
 int mfcfd_get_residual(mfcfd_ctx* c, mfcfd_residual* out) {
     if (!c || !out) return fail(MFCFD_EINVAL, "null argument");
     out->res_old = c->res_old;
@@ -1214,7 +1347,7 @@ int mfcfd_time_phases(mfcfd_ctx* c, int t0, int n_iters, double* out_ms) {
             launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
             launch_timestep(g, s, c->fp, c->stream);
         } else {
-            launch_qderiv_timestep(g, s, c->fp, c->stream);
+            launch_qderiv_timestep(g, s, c->fp, c->stream, c->shapes.qderiv);
         }
         hipEventRecord(ev[2], c->stream);
         if (c->params.flux_order == MFCFD_FLUX_LISTS) {

@@ -23,6 +23,18 @@ struct TypeLists {
     const int* generic;
 };
 
+// launch shape of a gather kernel: block size and the minimum waves per
+// SIMD its compiled variant targets (its register cap); 0 selects the default
+struct KernelShape {
+    int block = 0;
+    int waves = 0;
+};
+// shapes the autotuner chooses (defaults until it runs)
+struct LaunchShapes {
+    KernelShape flux_interior, flux_wall, qderiv;
+    int sweep_div = 1;   // sync-free sweeps use the resident grid / sweep_div
+};
+
 struct SweepSchedule {
     int n_levels_fwd, n_levels_bwd;
     const int* fwd_ptr;      // host copy: level l covers fwd_points[fwd_ptr[l] .. fwd_ptr[l+1])
@@ -47,9 +59,11 @@ struct SweepAsync {
 void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, hipStream_t st);
 // boundary permutations: dst[k*m + c] = src[perm[k]*m + c] and the inverse
 void launch_gather(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
-// precision conversions of a flat array (mixed form uploads and downloads)
-void launch_to_float(float* dst, const double* src, long long n, hipStream_t st);
-void launch_to_double(double* dst, const float* src, long long n, hipStream_t st);
+// q (m = 5, off = kQOff), qm (10, kQmOff) or dq (15, kDqOff) of every point
+// into / out of the records (FP32 records when mixed); src/dst hold m
+// values per point in device order
+void launch_rec_put(const DevState& s, bool mixed, const double* src, int n, int m, int off, hipStream_t st);
+void launch_rec_get(const DevState& s, bool mixed, double* dst, int n, int m, int off, hipStream_t st);
 void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
 
 // compute_conserved_vector.f90 / q_variables.f90
@@ -60,11 +74,13 @@ void launch_conserved_and_q(const DevGeom& g, const DevState& s, const FlowParam
 void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams& p, double* temp,
                           hipStream_t st);
 void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
-void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st);
+void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st,
+                            const KernelShape& ks = KernelShape());
 // flux_residual.f90; part bit 0: fused interior, bit 1: fused wall, bit 2: list-by-list
 constexpr int kFluxAll = 7;
 void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p,
-                          const TypeLists& tl, hipStream_t st, int parts = kFluxAll);
+                          const TypeLists& tl, hipStream_t st, int parts = kFluxAll,
+                          const LaunchShapes& ls = LaunchShapes());
 // reference-order flux residual of every point of a class (parity path)
 void launch_flux_residual_reference(const DevGeom& g, const DevState& s, const FlowParams& p,
                                     const TypeLists& tl, hipStream_t st);

@@ -21,10 +21,6 @@
 #include "kernels.h"
 #include "qlskum_math.h"
 
-// minimum waves per SIMD the fused kernels are compiled for (register cap)
-#ifndef MFCFD_FUSED_WAVES
-#define MFCFD_FUSED_WAVES 2
-#endif
 
 
 namespace mfcfd {
@@ -157,20 +153,23 @@ __device__ __forceinline__ void wall_axis(const EndState<T>& ei, const EndState<
 // T: arithmetic of the reconstruction and fluxes (double: the FP64 form;
 // float: the mixed form, reading the FP32 q, qm, dq); geometry, Cramer
 // weights and the sums stay FP64.
-template <int CLASS, bool UW, typename T>
-__global__ void __launch_bounds__(kBlock, MFCFD_FUSED_WAVES) k_flux_fused(DevGeom g, DevState s, FlowParams fp,
-                                                                      const int* idx, int count) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+// WAVES: minimum waves per SIMD the kernel is compiled for (its register
+// cap); the launch picks one of the compiled variants and a block size
+template <int CLASS, bool UW, typename T, int WAVES>
+__global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevState s, FlowParams fp, const int* idx,
+                                                             int count) {
+    const int m = xcd_index();
     if (m >= count) return;
-    const QArrays<T> qa = qarrays<T>(s);
+    const T* recs = qrecs<T>(s);
     const int i = idx[m];
     const Frame<double> f = load_frame(g.frame, i);
     const Frame<T> ft = frame_as<T>(f);
     const T pi = T(fp.pi);
     const double4 pi4 = g.pos[i];
-    const T* qip = qa.q + 5 * (long long)i;
-    const T* dqi = qa.dq + 15 * (long long)i;
-    const T* qmi = qa.qm + 10 * (long long)i;
+    const T* ri = recs + kQRec * (long long)i;
+    const T* qip = ri + kQOff;
+    const T* dqi = ri + kDqOff;
+    const T* qmi = ri + kQmOff;
     const T epsi2_i = T(limiter_epsi2(fp.vl_const, pi4.w));
     const double* cf = g.coef + 18 * (long long)i;
     const int nb = g.nbhs[i];
@@ -189,15 +188,15 @@ __global__ void __launch_bounds__(kBlock, MFCFD_FUSED_WAVES) k_flux_fused(DevGeo
         const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
         const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
         const double wgt = wls_weight<UW>(dels, delt, deln, fp.power);
-        T prim[5], qi[5], qk[5];
+        T prim[5], qi[5];
 #pragma unroll
         for (int r = 0; r < 5; ++r) qi[r] = qip[r];
         reconstruct_end<T>(qi, dqi, qmi, epsi2_i, T(delx), T(dely), T(delz), prim);
         const EndState<T> ei = end_state(prim, ft, pi);
-#pragma unroll
-        for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
-        reconstruct_end<T>(qk, qa.dq + 15 * (long long)k, qa.qm + 10 * (long long)k,
-                           T(limiter_epsi2(fp.vl_const, pk.w)), T(delx), T(dely), T(delz), prim);
+        T rk[30];
+        load_qrec<T>(recs + kQRec * (long long)k, rk);
+        reconstruct_end<T>(rk + kQOff, rk + kDqOff, rk + kQmOff, T(limiter_epsi2(fp.vl_const, pk.w)), T(delx),
+                           T(dely), T(delz), prim);
         const EndState<T> ek = end_state(prim, ft, pi);
         if (CLASS == 0) {
             interior_axis<0>(ei, ek, bits, cf, wgt, dels, delt, deln, acc);
@@ -223,30 +222,37 @@ __global__ void __launch_bounds__(kBlock, MFCFD_FUSED_WAVES) k_flux_fused(DevGeo
 
 template <int CLASS, typename T>
 void launch_class(const DevGeom& g, const DevState& s, const FlowParams& p, const int* idx, int count,
-                  hipStream_t st) {
+                  const KernelShape& ks, hipStream_t st) {
     if (count <= 0) return;
-    const dim3 grid(grid_for(count, kBlock));
-    if (p.power == 0.0)
-        hipLaunchKernelGGL((k_flux_fused<CLASS, true, T>), grid, dim3(kBlock), 0, st, g, s, p, idx, count);
-    else
-        hipLaunchKernelGGL((k_flux_fused<CLASS, false, T>), grid, dim3(kBlock), 0, st, g, s, p, idx, count);
+    const int block = ks.block > 0 && ks.block <= kBlock ? ks.block : kBlock;
+    const dim3 grid(grid_for(count, block)), blk(block);
+    if (p.power != 0.0) {
+        hipLaunchKernelGGL((k_flux_fused<CLASS, false, T, 2>), grid, blk, 0, st, g, s, p, idx, count);
+        return;
+    }
+    switch (ks.waves) {
+        case 1: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 1>), grid, blk, 0, st, g, s, p, idx, count); break;
+        case 3: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 3>), grid, blk, 0, st, g, s, p, idx, count); break;
+        case 4: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 4>), grid, blk, 0, st, g, s, p, idx, count); break;
+        default: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 2>), grid, blk, 0, st, g, s, p, idx, count); break;
+    }
 }
 
 }  // namespace
 
 void launch_flux_fused(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
-                       hipStream_t st, int parts, bool mixed) {
+                       hipStream_t st, int parts, bool mixed, const LaunchShapes& ls) {
     if (parts & 1) {
         if (mixed)
-            launch_class<0, float>(g, s, p, tl.fused_interior, tl.n_fused_interior, st);
+            launch_class<0, float>(g, s, p, tl.fused_interior, tl.n_fused_interior, ls.flux_interior, st);
         else
-            launch_class<0, double>(g, s, p, tl.fused_interior, tl.n_fused_interior, st);
+            launch_class<0, double>(g, s, p, tl.fused_interior, tl.n_fused_interior, ls.flux_interior, st);
     }
     if (parts & 2) {
         if (mixed)
-            launch_class<1, float>(g, s, p, tl.fused_wall, tl.n_fused_wall, st);
+            launch_class<1, float>(g, s, p, tl.fused_wall, tl.n_fused_wall, ls.flux_wall, st);
         else
-            launch_class<1, double>(g, s, p, tl.fused_wall, tl.n_fused_wall, st);
+            launch_class<1, double>(g, s, p, tl.fused_wall, tl.n_fused_wall, ls.flux_wall, st);
     }
 }
 

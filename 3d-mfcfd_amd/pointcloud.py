@@ -704,6 +704,39 @@ def config_cloud(name: str, workers: int = -1) -> PointCloud:
                            order=c["order"], workers=workers)
 
 
+# batched configuration: many independent clouds iterated together
+BATCH = dict(n_clouds=64, n_min=20000, n_max=500000, aspect=(2.0, 6.0), mach=(0.3, 0.85),
+             aoa=(-5.0, 10.0), seed=5, cfl=0.2, mode="explicit")
+
+
+def batch_plan(n_clouds: int = BATCH["n_clouds"], seed: int = BATCH["seed"],
+               n_min: int = BATCH["n_min"], n_max: int = BATCH["n_max"]):
+    """Sizes (log-uniform in [n_min, n_max]), body axes (aspect a:b uniform
+    in 2..6 with a = 0.5 and thickness c = 0.3 b, as the 0.5/0.167/0.05
+    body), Mach and angle of attack of each cloud, all from one seed.  Each
+    cloud has 8% wall, 4% outer and 88% interior points, the proportions of
+    the 50k configuration."""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_clouds)).astype(np.int64)
+    aspect = rng.uniform(*BATCH["aspect"], n_clouds)
+    mach = rng.uniform(*BATCH["mach"], n_clouds)
+    aoa = rng.uniform(*BATCH["aoa"], n_clouds)
+    plan = []
+    for b in range(n_clouds):
+        n_wall = int(round(0.08 * n[b]))
+        n_outer = int(round(0.04 * n[b]))
+        b_ax = 0.5 / aspect[b]
+        plan.append(dict(n_wall=n_wall, n_outer=n_outer, n_interior=int(n[b]) - n_wall - n_outer,
+                         axes=(0.5, b_ax, 0.3 * b_ax), mach=float(mach[b]), aoa=float(aoa[b]),
+                         seed=seed * 1000 + b))
+    return plan
+
+
+def batch_clouds(plan, workers: int = -1):
+    return [ellipsoid_cloud(p["n_wall"], p["n_outer"], p["n_interior"], seed=p["seed"], axes=p["axes"],
+                            order="morton", workers=workers) for p in plan]
+
+
 if __name__ == "__main__":  # pragma: no cover - manual deck writer
     import argparse
     ap = argparse.ArgumentParser(description="write a synthetic ellipsoid deck")

@@ -45,6 +45,19 @@ class Params(C.Structure):
                 ("max_iters", C.c_int), ("flux_order", C.c_int), ("sweep_schedule", C.c_int)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("flux_interior_block", C.c_int), ("flux_interior_waves", C.c_int),
+                ("flux_wall_block", C.c_int), ("flux_wall_waves", C.c_int),
+                ("qderiv_block", C.c_int), ("qderiv_waves", C.c_int), ("sweep_div", C.c_int),
+                ("flux_interior_ms", C.c_double), ("flux_wall_ms", C.c_double),
+                ("qderiv_ms", C.c_double), ("sweeps_ms", C.c_double),
+                ("flux_interior_default_ms", C.c_double), ("flux_wall_default_ms", C.c_double),
+                ("qderiv_default_ms", C.c_double), ("sweeps_default_ms", C.c_double)]
+
+    def as_dict(self):
+        return {nm: getattr(self, nm) for nm, _ in self._fields_}
+
+
 class Geometry(C.Structure):
     _fields_ = [("n", C.c_int), ("x", _dp), ("y", _dp), ("z", _dp), ("tan1", _dp),
                 ("tan2", _dp), ("nor", _dp), ("min_dist", _dp), ("status", _ip),
@@ -70,7 +83,7 @@ EXPORTS = ["mfcfd_create", "mfcfd_destroy", "mfcfd_last_error", "mfcfd_abi_versi
            "mfcfd_run", "mfcfd_get_residual", "mfcfd_synchronize", "mfcfd_sweep_info",
            "mfcfd_time_phases", "mfcfd_point_classes", "mfcfd_point_class_map",
            "mfcfd_kernel_timing", "mfcfd_kernel_time", "mfcfd_bind_batch", "mfcfd_batch_info",
-           "mfcfd_run_graph"]
+           "mfcfd_run_graph", "mfcfd_autotune", "mfcfd_set_tuning"]
 PHASES = ["conserved_q", "qderiv_timestep", "flux_interior", "flux_boundary", "sweeps",
           "state_update"]
 
@@ -110,6 +123,8 @@ def lib():
         L.mfcfd_bind_batch.argtypes = [vp, C.c_int, C.POINTER(Geometry), _dp, _dp]
         L.mfcfd_batch_info.argtypes = [vp, _ip, _ip]
         L.mfcfd_run_graph.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+        L.mfcfd_autotune.argtypes = [vp, C.c_int, C.POINTER(Tuning)]
+        L.mfcfd_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
     return _lib
 
@@ -314,6 +329,17 @@ class Solver:
         inst = np.zeros((n_iters, ni))
         _check(lib().mfcfd_run_graph(self._ctx, t0, n_iters, _d(rn), _d(inst)))
         return rn, inst
+
+    def autotune(self, repeats=3):
+        """Time the compiled launch-shape variants on this cloud and keep the
+        fastest (mfcfd_autotune); returns the choices and times."""
+        t = Tuning()
+        _check(lib().mfcfd_autotune(self._ctx, repeats, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, tuning):
+        t = Tuning(**{k: v for k, v in tuning.items() if k in dict(Tuning._fields_)})
+        _check(lib().mfcfd_set_tuning(self._ctx, C.byref(t)))
 
     def batch_info(self):
         k = C.c_int()

@@ -97,12 +97,95 @@ def copy_probe(mib):
     log(f"copy probe: 3 launches of {mib} MiB read + {mib} MiB written")
 
 
+def bench_batch(args):
+    """64 independent clouds (log-uniform 20k..500k points, aspect 2..6,
+    Mach 0.3..0.85, AoA -5..10 deg) in one context, the iteration captured
+    as a HIP graph, launch shapes from the autotuner."""
+    cfg = pc.BATCH
+    plan = pc.batch_plan()
+    log(f"generating {len(plan)} clouds")
+    t0 = time.perf_counter()
+    clouds = pc.batch_clouds(plan, workers=16)
+    n = int(sum(c.n for c in clouds))
+    log(f"{n} points in {time.perf_counter() - t0:.1f} s")
+    machs = [p["mach"] for p in plan]
+    aoas = [p["aoa"] for p in plan]
+    p0 = np.concatenate([np.tile(pc.freestream_prim(m, a), (c.n, 1)) for c, m, a in zip(clouds, machs, aoas)])
+    W, K = args.warmup, args.steps
+    legs = {}
+    for precision in (["fp64"] if args.no_mixed else ["fp64", "mixed"]):
+        bs = solver.BatchSolver(clouds, machs, aoas, cfg["cfl"], mode=cfg["mode"], precision=precision)
+        bs.upload("prim", p0)
+        tuning = None if args.no_tune else bs.autotune()
+        if W > 0:
+            bs.run_graph(1, W)
+        bs.synchronize()
+        t0 = time.perf_counter()
+        _, inst = bs.run_graph(W + 1, K)
+        bs.synchronize()
+        el = time.perf_counter() - t0
+        # the dominant kernel, timed with events over two eager iterations
+        bs.kernel_timing(True)
+        bs.run(W + K + 1, 2)
+        fms, fl = bs.kernel_time()
+        bs.kernel_timing(False)
+        cls = bs.point_class_map()
+        prim = bs.download("prim")
+        bs.close()
+        legs[precision] = dict(el=el, inst=inst, tuning=tuning, flux_ms=fms / max(fl, 1), cls=cls, prim=prim)
+        log(f"{precision}: {el / K * 1000:.3f} ms/iter, {n * K / el / 1e6:.2f} Mpoint-iter/s")
+    f64 = legs["fp64"]
+    value = n * K / f64["el"] / 1e6
+    status = np.concatenate([c.status for c in clouds])
+    cnt = np.concatenate([solver.split_stencils(c)[0] for c in clouds], axis=1)
+    nbhs = np.concatenate([c.nbhs for c in clouds])
+    flux_bytes = int(roofline.flux_bytes_per_point(status, cnt, nbhs)[f64["cls"] == 0].sum())
+    achieved = flux_bytes / (f64["flux_ms"] * 1e-3) / 1e9
+    last = f64["inst"][-1]
+    mixed = None
+    if "mixed" in legs:
+        mx = legs["mixed"]
+        offs = np.concatenate([[0], np.cumsum([c.n for c in clouds])])
+        errs = []
+        for b, c in enumerate(clouds):
+            w = offs[b] + np.flatnonzero(c.status == pc.STATUS_WALL)
+            a = postprocess.wall_cp(f64["prim"][w], machs[b])
+            m = postprocess.wall_cp(mx["prim"][w], machs[b])
+            errs.append(float(np.max(np.abs(m - a)) / np.max(np.abs(a))))
+        vm = n * K / mx["el"] / 1e6
+        mixed = {"value": round(vm, 3), "ms_per_step": round(mx["el"] / K * 1000, 4),
+                 "speedup_vs_fp64": round(vm / value, 3), "wall_cp_err_vs_fp64_max": max(errs),
+                 "tuning": mx["tuning"]}
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpoint-iter/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": round(f64["el"] / K * 1000.0, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"c5_batch: {len(clouds)} ellipsoid clouds, {n} points in all "
+                               f"(log-uniform {cfg['n_min']}..{cfg['n_max']}), aspect "
+                               f"{cfg['aspect'][0]}..{cfg['aspect'][1]}, Mach {cfg['mach'][0]}..{cfg['mach'][1]}, "
+                               f"AoA {cfg['aoa'][0]}..{cfg['aoa'][1]} deg, FP64 {cfg['mode']}, CFL {cfg['cfl']}, "
+                               f"one HIP graph per iteration",
+                   "points": n, "clouds": len(clouds), "mode": cfg["mode"], "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_flux_fused<0,true,double> (interior points on the fused path)",
+                     "bytes_per_launch": flux_bytes, "launch_ms": round(f64["flux_ms"], 4)},
+        "cpu_baseline": None,
+        "mixed": mixed,
+        "tuning": f64["tuning"],
+        "instance_residual_last": {"min": float(last.min()), "max": float(last.max()),
+                                   "finite": int(np.isfinite(last).sum())},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c2_2m", choices=sorted(pc.CONFIGS))
+    ap.add_argument("--config", default="c2_2m", choices=sorted(pc.CONFIGS) + ["c5_batch"])
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--phase-iters", type=int, default=3)
@@ -117,6 +200,9 @@ def main():
               f"the q-LSKUM iteration on a single MI355X)", file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "c5_batch":
+        bench_batch(args)
+        return
     cfg = pc.CONFIGS[args.config]
     log(f"generating {args.config}")
     t0 = time.perf_counter()
@@ -129,6 +215,10 @@ def main():
                       split=(cnt, lists), alias=alias)
     s.upload("prim", prim0)
     log(f"bound; sweep schedule {s.sweep_info()}")
+    tuning = None
+    if not args.no_tune:
+        tuning = s.autotune()
+        log(f"autotune: {tuning}")
 
     W, K = args.warmup, args.steps
     if W > 0:
@@ -172,6 +262,8 @@ def main():
         m = solver.Solver(cloud, cfg["mach"], cfg["aoa"], cfg["cfl"], mode=cfg["mode"],
                           precision="mixed", split=(cnt, lists), alias=alias)
         m.upload("prim", prim0)
+        if not args.no_tune:
+            m.autotune()
         if W > 0:
             m.run(1, W)
         m.synchronize()
@@ -213,6 +305,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "tuning": tuning,
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
         "step_algorithmic_gbs": round(step_gbs, 2),
         "step_hbm_frac": round(step_gbs / roofline.HBM_PEAK_GBS, 5),

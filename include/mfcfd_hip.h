@@ -197,6 +197,27 @@ int mfcfd_run(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, double 
  * may be NULL. */
 int mfcfd_run_graph(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, double *inst_res_hist);
 
+/* launch-shape autotuner (no counterpart in the reference).  Times, on the
+ * bound cloud(s) and the current state, every compiled variant of the
+ * gather kernels -- block size 64/128/256 x minimum waves per SIMD (the
+ * register cap: 2/4/6 for the q-derivative kernel, 1..4 for the fused flux
+ * kernels) -- and, in implicit mode, the sync-free sweeps on 1, 1/2 or 1/4
+ * of the resident grid; keeps the fastest.  Results do not depend on the
+ * shapes.  It recomputes U, q, qm, dq, delt, flux_res and (implicit)
+ * delUp/delUn from prim as one iteration's first steps would; prim is not
+ * touched. */
+typedef struct mfcfd_tuning {
+    int flux_interior_block, flux_interior_waves;
+    int flux_wall_block, flux_wall_waves;
+    int qderiv_block, qderiv_waves;
+    int sweep_div;
+    double flux_interior_ms, flux_wall_ms, qderiv_ms, sweeps_ms;   /* chosen shapes */
+    double flux_interior_default_ms, flux_wall_default_ms, qderiv_default_ms, sweeps_default_ms;
+} mfcfd_tuning;
+int mfcfd_autotune(mfcfd_ctx *ctx, int repeats, mfcfd_tuning *out);
+/* apply shapes chosen earlier (times ignored) */
+int mfcfd_set_tuning(mfcfd_ctx *ctx, const mfcfd_tuning *in);
+
 int mfcfd_get_residual(mfcfd_ctx *ctx, mfcfd_residual *out);
 int mfcfd_synchronize(mfcfd_ctx *ctx);
 

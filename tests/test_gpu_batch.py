@@ -81,3 +81,28 @@ def test_graph_replays_match_eager_iterations():
     inst = np.concatenate([inst1[:, 0], inst2[:, 0]])
     nz = rn_e != 0
     assert np.max(np.abs(inst[nz] - rn_e[nz]) / rn_e[nz]) < 1e-13
+
+
+def test_autotuned_shapes_leave_results_unchanged():
+    """The tuner picks block sizes, register caps and the sweep grid; none
+    changes any point's arithmetic, so tuned and default runs agree bit for
+    bit (implicit, 20k shell cloud)."""
+    note("[gpu] autotune: tuned vs default launch shapes")
+    cl = pc.ellipsoid_cloud(1600, 800, 17600, seed=21, order="morton")
+    mach, aoa, cfl = 0.84, 3.06, 1.0
+    p0 = prim0(cl, mach, aoa)
+    out = []
+    for tune in (False, True):
+        s = solver.Solver(cl, mach, aoa, cfl, mode="implicit")
+        s.upload("prim", p0)
+        if tune:
+            t = s.autotune(2)
+            note(f"      tuning {t}")
+            assert t["flux_interior_block"] in (64, 128, 256)
+            assert t["flux_interior_ms"] <= t["flux_interior_default_ms"]
+        rn, _ = s.run(1, 6)
+        out.append((rn, s.download("prim")))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+

@@ -128,8 +128,8 @@ struct mfcfd_ctx {
     DevBuf<int> idx_fused_interior, idx_fused_wall, idx_generic;
     TypeLists tl{};
     // state
-    DevBuf<double> prim, U, qrec, delt, flux_res, delUp, delUn, dinv, temp, staging, staging2;
-    DevBuf<float> qrec32;   // mixed precision records
+    DevBuf<double> prim, U, q, qm, dq, delt, flux_res, delUp, delUn, dinv, temp, staging;
+    DevBuf<float> q32, qm32, dq32;   // mixed precision
     // residual
     DevBuf<double> partial;
     DevBuf<int> partial_pos;
@@ -196,18 +196,22 @@ struct mfcfd_ctx {
         DevState s;
         s.prim = prim.p;
         s.U = U.p;
-        s.qrec = qrec.p;
+        s.q = q.p;
+        s.qm = qm.p;
+        s.dq = dq.p;
         s.delt = delt.p;
         s.flux_res = flux_res.p;
         s.delUp = delUp.p;
         s.delUn = delUn.p;
         s.dinv = dinv.p;
-        s.qrec32 = qrec32.p;
+        s.q32 = q32.p;
+        s.qm32 = qm32.p;
+        s.dq32 = dq32.p;
         return s;
     }
     void release_all() {
-        DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &qrec, &delt, &flux_res, &delUp, &delUn,
-                                 &dinv, &temp, &staging, &staging2, &partial, &hist};
+        DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &q, &qm, &dq, &delt, &flux_res, &delUp, &delUn,
+                                 &dinv, &temp, &staging, &partial, &hist};
         for (auto* b : dbl) b->release();
         DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
@@ -216,7 +220,9 @@ struct mfcfd_ctx {
         for (auto* b : ints) b->release();
         fwd_dep_ptr.release();
         bwd_dep_ptr.release();
-        qrec32.release();
+        q32.release();
+        qm32.release();
+        dq32.release();
         sweep_ctr.release();
         DevBuf<int>* ib[] = {&inst_id, &inst_beg, &inst_end, &hist_ctr};
         for (auto* b : ib) b->release();
@@ -258,13 +264,13 @@ struct ArrayDesc {
     int per_point;
 };
 
-// q, qm and dq live in the per-point records: field offset and width
-bool record_field(int id, int* off, int* m) {
+// FP32 array the mixed form keeps for an array id (null: none)
+float* mixed_copy_of(mfcfd_ctx* c, int id) {
     switch (id) {
-        case MFCFD_ARR_Q: *off = kQOff; *m = 5; return true;
-        case MFCFD_ARR_QM: *off = kQmOff; *m = 10; return true;
-        case MFCFD_ARR_DQ: *off = kDqOff; *m = 15; return true;
-        default: return false;
+        case MFCFD_ARR_Q: return c->q32.p;
+        case MFCFD_ARR_QM: return c->qm32.p;
+        case MFCFD_ARR_DQ: return c->dq32.p;
+        default: return nullptr;
     }
 }
 
@@ -272,9 +278,9 @@ ArrayDesc array_of(mfcfd_ctx* c, int id) {
     switch (id) {
         case MFCFD_ARR_PRIM: return {&c->prim, 5};
         case MFCFD_ARR_U: return {&c->U, 5};
-        case MFCFD_ARR_Q: return {&c->staging2, 5};
-        case MFCFD_ARR_QM: return {&c->staging2, 10};
-        case MFCFD_ARR_DQ: return {&c->staging2, 15};
+        case MFCFD_ARR_Q: return {&c->q, 5};
+        case MFCFD_ARR_QM: return {&c->qm, 10};
+        case MFCFD_ARR_DQ: return {&c->dq, 15};
         case MFCFD_ARR_DELT: return {&c->delt, 1};
         case MFCFD_ARR_FLUX_RES: return {&c->flux_res, 5};
         case MFCFD_ARR_DELUP: return {&c->delUp, 5};
@@ -739,18 +745,23 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     if ((rc = launch_error("stencil coefficients"))) return rc;
 
     // flow state (zero until uploaded; initial_conditions.f90:23-24 zero delUp/delUn)
-    DevBuf<double>* st5[] = {&c->prim, &c->U, &c->flux_res, &c->delUp, &c->delUn};
+    DevBuf<double>* st5[] = {&c->prim, &c->U, &c->q, &c->flux_res, &c->delUp, &c->delUn};
     for (auto* b : st5) {
         if (b->alloc(5ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "state arrays");
         HIP_TRY(hipMemsetAsync(b->p, 0, 5ULL * n * sizeof(double), st));
     }
-    if (c->qrec.alloc((size_t)kQRec * n) != hipSuccess || c->staging2.alloc(15ULL * n) != hipSuccess ||
+    if (c->qm.alloc(10ULL * n) != hipSuccess || c->dq.alloc(15ULL * n) != hipSuccess ||
         c->delt.alloc((size_t)n) != hipSuccess || c->dinv.alloc((size_t)n) != hipSuccess ||
         c->staging.alloc(15ULL * n) != hipSuccess)
         return fail(MFCFD_ENOMEM, "state arrays");
-    HIP_TRY(hipMemsetAsync(c->qrec.p, 0, (size_t)kQRec * n * sizeof(double), st));
-    if (c->qrec32.alloc((size_t)kQRec * n) != hipSuccess) return fail(MFCFD_ENOMEM, "FP32 q records");
-    HIP_TRY(hipMemsetAsync(c->qrec32.p, 0, (size_t)kQRec * n * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->qm.p, 0, 10ULL * n * sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(c->dq.p, 0, 15ULL * n * sizeof(double), st));
+    if (c->q32.alloc(5ULL * n) != hipSuccess || c->qm32.alloc(10ULL * n) != hipSuccess ||
+        c->dq32.alloc(15ULL * n) != hipSuccess)
+        return fail(MFCFD_ENOMEM, "FP32 q arrays");
+    HIP_TRY(hipMemsetAsync(c->q32.p, 0, 5ULL * n * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->qm32.p, 0, 10ULL * n * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->dq32.p, 0, 15ULL * n * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->delt.p, 0, (size_t)n * sizeof(double), st));
     HIP_TRY(hipMemsetAsync(c->dinv.p, 0, (size_t)n * sizeof(double), st));
     if (c->fp.inner_iterations > 0 && c->temp.alloc(15ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "temp");
@@ -1008,8 +1019,9 @@ int mfcfd_upload(mfcfd_ctx* c, int id, const double* host) {
     const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
     HIP_TRY(hipMemcpyAsync(c->staging.p, host, bytes, hipMemcpyHostToDevice, c->stream));
     launch_gather(a.buf->p, c->staging.p, c->perm.p, c->n, a.per_point, c->stream);
-    int off = 0, m = 0;
-    if (record_field(id, &off, &m)) launch_rec_put(c->state(), c->fp.mixed != 0, a.buf->p, c->n, m, off, c->stream);
+    // mixed form: the double array is staging for the FP32 copy the kernels read
+    if (float* f32 = c->fp.mixed ? mixed_copy_of(c, id) : nullptr)
+        launch_to_float(f32, a.buf->p, (long long)a.per_point * c->n, c->stream);
     if ((rc = launch_error("upload permutation"))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MFCFD_OK;
@@ -1021,8 +1033,8 @@ int mfcfd_download(mfcfd_ctx* c, int id, double* host) {
     ArrayDesc a = array_of(c, id);
     if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
     const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
-    int off = 0, m = 0;
-    if (record_field(id, &off, &m)) launch_rec_get(c->state(), c->fp.mixed != 0, a.buf->p, c->n, m, off, c->stream);
+    if (float* f32 = c->fp.mixed ? mixed_copy_of(c, id) : nullptr)
+        launch_to_double(a.buf->p, f32, (long long)a.per_point * c->n, c->stream);
     launch_scatter(c->staging.p, a.buf->p, c->perm.p, c->n, a.per_point, c->stream);
     if ((rc = launch_error("download permutation"))) return rc;
     HIP_TRY(hipMemcpyAsync(host, c->staging.p, bytes, hipMemcpyDeviceToHost, c->stream));

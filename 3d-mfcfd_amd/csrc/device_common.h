@@ -54,58 +54,6 @@ __device__ __forceinline__ Frame<double> load_frame(const double* frame, int k) 
     return f;
 }
 
-// the 30 used values of a point's q/qm/dq record with 16-byte loads
-template <typename T>
-__device__ __forceinline__ void load_qrec(const T* r, T v[30]);
-template <>
-__device__ __forceinline__ void load_qrec<double>(const double* r, double v[30]) {
-    const double2* p = reinterpret_cast<const double2*>(r);
-#pragma unroll
-    for (int j = 0; j < 15; ++j) {
-        const double2 t = p[j];
-        v[2 * j] = t.x;
-        v[2 * j + 1] = t.y;
-    }
-}
-template <>
-__device__ __forceinline__ void load_qrec<float>(const float* r, float v[30]) {
-    const float4* p = reinterpret_cast<const float4*>(r);
-#pragma unroll
-    for (int j = 0; j < 7; ++j) {
-        const float4 t = p[j];
-        v[4 * j] = t.x;
-        v[4 * j + 1] = t.y;
-        v[4 * j + 2] = t.z;
-        v[4 * j + 3] = t.w;
-    }
-    const float2 t = reinterpret_cast<const float2*>(r)[14];
-    v[28] = t.x;
-    v[29] = t.y;
-}
-
-// q(1:5) of a record
-template <typename T>
-__device__ __forceinline__ void load_q5(const T* r, T q[5]);
-template <>
-__device__ __forceinline__ void load_q5<double>(const double* r, double q[5]) {
-    const double2 a = reinterpret_cast<const double2*>(r)[0];
-    const double2 b = reinterpret_cast<const double2*>(r)[1];
-    q[0] = a.x;
-    q[1] = a.y;
-    q[2] = b.x;
-    q[3] = b.y;
-    q[4] = r[4];
-}
-template <>
-__device__ __forceinline__ void load_q5<float>(const float* r, float q[5]) {
-    const float4 a = reinterpret_cast<const float4*>(r)[0];
-    q[0] = a.x;
-    q[1] = a.y;
-    q[2] = a.z;
-    q[3] = a.w;
-    q[4] = r[4];
-}
-
 // least-squares weight 1/dist**power; UNIT_W selects the power = 0 kernels,
 // where the weight is exactly 1 and no power function is compiled in
 template <bool UNIT_W>

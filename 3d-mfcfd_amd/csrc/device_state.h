@@ -58,37 +58,40 @@ struct DevGeom {
     const double* inst_fs;
 };
 
-// q, qm and dq of one point packed in a record of kQRec values: q(1:5) at
-// kQOff, qm(1:2,1:5) at kQmOff (max, min per component), dq(1:3,1:5) at
-// kDqOff (x, y, z per component), two unused.  A neighbour's whole
-// reconstruction input is then two aligned 128-byte lines in FP64 (one in
-// FP32), read with 16-byte loads.
-constexpr int kQRec = 32;
-constexpr int kQOff = 0;
-constexpr int kQmOff = 5;
-constexpr int kDqOff = 15;
-
 struct DevState {
     double* prim;
     double* U;
-    double* qrec;     // [kQRec * n] FP64 records
+    double* q;
+    double* qm;
+    double* dq;
     double* delt;
     double* flux_res;
     double* delUp;
     double* delUn;
     double* dinv;
-    float* qrec32;    // [kQRec * n] FP32 records of the mixed form
+    // mixed precision: q, qm, dq held in FP32 (same layouts)
+    float* q32;
+    float* qm32;
+    float* dq32;
+};
+
+// q-variable arrays of one precision
+template <typename T>
+struct QArrays {
+    T* q;
+    T* qm;
+    T* dq;
 };
 
 template <typename T>
-__host__ __device__ inline T* qrecs(const DevState& s);
+__host__ __device__ inline QArrays<T> qarrays(const DevState& s);
 template <>
-__host__ __device__ inline double* qrecs<double>(const DevState& s) {
-    return s.qrec;
+__host__ __device__ inline QArrays<double> qarrays<double>(const DevState& s) {
+    return {s.q, s.qm, s.dq};
 }
 template <>
-__host__ __device__ inline float* qrecs<float>(const DevState& s) {
-    return s.qrec32;
+__host__ __device__ inline QArrays<float> qarrays<float>(const DevState& s) {
+    return {s.q32, s.qm32, s.dq32};
 }
 
 struct FlowParams {

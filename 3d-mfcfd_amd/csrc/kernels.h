@@ -59,11 +59,9 @@ struct SweepAsync {
 void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, hipStream_t st);
 // boundary permutations: dst[k*m + c] = src[perm[k]*m + c] and the inverse
 void launch_gather(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
-// q (m = 5, off = kQOff), qm (10, kQmOff) or dq (15, kDqOff) of every point
-// into / out of the records (FP32 records when mixed); src/dst hold m
-// values per point in device order
-void launch_rec_put(const DevState& s, bool mixed, const double* src, int n, int m, int off, hipStream_t st);
-void launch_rec_get(const DevState& s, bool mixed, double* dst, int n, int m, int off, hipStream_t st);
+// precision conversions of a flat array (mixed form uploads and downloads)
+void launch_to_float(float* dst, const double* src, long long n, hipStream_t st);
+void launch_to_double(double* dst, const float* src, long long n, hipStream_t st);
 void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
 
 // compute_conserved_vector.f90 / q_variables.f90

@@ -160,16 +160,15 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
                                                              int count) {
     const int m = xcd_index();
     if (m >= count) return;
-    const T* recs = qrecs<T>(s);
+    const QArrays<T> qa = qarrays<T>(s);
     const int i = idx[m];
     const Frame<double> f = load_frame(g.frame, i);
     const Frame<T> ft = frame_as<T>(f);
     const T pi = T(fp.pi);
     const double4 pi4 = g.pos[i];
-    const T* ri = recs + kQRec * (long long)i;
-    const T* qip = ri + kQOff;
-    const T* dqi = ri + kDqOff;
-    const T* qmi = ri + kQmOff;
+    const T* qip = qa.q + 5 * (long long)i;
+    const T* dqi = qa.dq + 15 * (long long)i;
+    const T* qmi = qa.qm + 10 * (long long)i;
     const T epsi2_i = T(limiter_epsi2(fp.vl_const, pi4.w));
     const double* cf = g.coef + 18 * (long long)i;
     const int nb = g.nbhs[i];
@@ -193,10 +192,11 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
         for (int r = 0; r < 5; ++r) qi[r] = qip[r];
         reconstruct_end<T>(qi, dqi, qmi, epsi2_i, T(delx), T(dely), T(delz), prim);
         const EndState<T> ei = end_state(prim, ft, pi);
-        T rk[30];
-        load_qrec<T>(recs + kQRec * (long long)k, rk);
-        reconstruct_end<T>(rk + kQOff, rk + kDqOff, rk + kQmOff, T(limiter_epsi2(fp.vl_const, pk.w)), T(delx),
-                           T(dely), T(delz), prim);
+        T qk[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
+        reconstruct_end<T>(qk, qa.dq + 15 * (long long)k, qa.qm + 10 * (long long)k,
+                           T(limiter_epsi2(fp.vl_const, pk.w)), T(delx), T(dely), T(delz), prim);
         const EndState<T> ek = end_state(prim, ft, pi);
         if (CLASS == 0) {
             interior_axis<0>(ei, ek, bits, cf, wgt, dels, delt, deln, acc);

@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
         q[2] = two_times_beta * u2;
         q[3] = two_times_beta * u3;
         q[4] = -two_times_beta;
-        QT* qo = qrecs<QT>(s) + kQRec * (long long)k + kQOff;
+        QT* qo = qarrays<QT>(s).q + 5 * (long long)k;
 #pragma unroll
         for (int r = 0; r < 5; ++r) qo[r] = QT(q[r]);
     }
@@ -132,8 +132,7 @@ template <bool DO_DQ, bool DO_DT, bool UW, typename QT, int WAVES = 4>
 __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, DevState s, FlowParams fp) {
     const int i = xcd_index();
     if (i >= g.n) return;
-    const QT* recs = qrecs<QT>(s);
-    QT* ri = qrecs<QT>(s) + kQRec * (long long)i;
+    const QArrays<QT> qa = qarrays<QT>(s);
     const double4 pi4 = g.pos[i];
     const int nb = g.nbhs[i];
     const int* cn = g.conn + kConnCols * (long long)i;
@@ -141,9 +140,9 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
     QT sx[5], sy[5], sz[5];
     Geo6<double> geo = {0, 0, 0, 0, 0, 0};
     if (DO_DQ) {
-        load_q5<QT>(ri, qi);
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
+            qi[r] = qa.q[5 * (long long)i + r];
             qmax[r] = qi[r];
             qmin[r] = qi[r];
             sx[r] = sy[r] = sz[r] = QT(0);
@@ -158,7 +157,8 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
         const double delz = pk.z - pi4.z;
         if (DO_DQ) {
             QT qk[5];
-            load_q5<QT>(recs + kQRec * (long long)k, qk);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 if (qk[r] > qmax[r]) qmax[r] = qk[r];
@@ -194,11 +194,11 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
         const double one_by_det = 1.0 / det;
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            ri[kQmOff + 2 * r + 0] = qmax[r];
-            ri[kQmOff + 2 * r + 1] = qmin[r];
+            qa.qm[(5 * (long long)i + r) * 2 + 0] = qmax[r];
+            qa.qm[(5 * (long long)i + r) * 2 + 1] = qmin[r];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                ri[kDqOff + 3 * r + c] =
+                qa.dq[(5 * (long long)i + r) * 3 + c] =
                     QT(cramer_num(geo, c, (double)sx[r], (double)sy[r], (double)sz[r]) * one_by_det);
         }
     }
@@ -231,12 +231,10 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_inner(DevGeom g, DevState s, 
         geo.zx = geo.zx + delz * delx * weights;
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            const double* ri = s.qrec + kQRec * (long long)i;
-            const double* rk = s.qrec + kQRec * (long long)k;
-            const double* dqi = ri + kDqOff + 3 * r;
-            const double* dqk = rk + kDqOff + 3 * r;
-            const double qti = ri[kQOff + r] - 0.5 * (delx * dqi[0] + dely * dqi[1] + delz * dqi[2]);
-            const double qtn = rk[kQOff + r] - 0.5 * (delx * dqk[0] + dely * dqk[1] + delz * dqk[2]);
+            const double* dqi = s.dq + (5 * (long long)i + r) * 3;
+            const double* dqk = s.dq + (5 * (long long)k + r) * 3;
+            const double qti = s.q[5 * (long long)i + r] - 0.5 * (delx * dqi[0] + dely * dqi[1] + delz * dqi[2]);
+            const double qtn = s.q[5 * (long long)k + r] - 0.5 * (delx * dqk[0] + dely * dqk[1] + delz * dqk[2]);
             const double t = qtn - qti;
             sx[r] = sx[r] + weights * delx * t;
             sy[r] = sy[r] + weights * dely * t;
@@ -252,22 +250,14 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_inner(DevGeom g, DevState s, 
             temp[(5 * (long long)i + r) * 3 + c] = cramer_num(geo, c, sx[r], sy[r], sz[r]) * one_by_det;
 }
 
-// a field of every point's record from / to an array of m values per point
-// (device order); T is the record type
-template <typename T>
-__global__ void k_rec_put(T* rec, const double* src, int n, int m, int off) {
-    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (e >= (long long)n * m) return;
-    const long long k = e / m;
-    rec[kQRec * k + off + (e - k * m)] = (T)src[e];
+__global__ void k_to_float(float* dst, const double* src, long long n) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
 }
 
-template <typename T>
-__global__ void k_rec_get(double* dst, const T* rec, int n, int m, int off) {
-    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (e >= (long long)n * m) return;
-    const long long k = e / m;
-    dst[e] = (double)rec[kQRec * k + off + (e - k * m)];
+__global__ void k_to_double(double* dst, const float* src, long long n) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i];
 }
 
 __global__ void k_copy(double* dst, const double* src, long long n) {
@@ -304,15 +294,14 @@ __device__ __forceinline__ Stencil dir_stencil(const DevGeom& g, int cls, int d,
 template <bool UW, typename T>
 __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
                            const Stencil& st, const Frame<double>& f, double G[5]) {
-    const T* recs = qrecs<T>(s);
-    const T* ri = recs + kQRec * (long long)i;
+    const QArrays<T> qa = qarrays<T>(s);
     const Frame<T> ft = frame_as<T>(f);
     Geo6<double> geo = {0, 0, 0, 0, 0, 0};
     double sx[5] = {0, 0, 0, 0, 0}, sy[5] = {0, 0, 0, 0, 0}, sz[5] = {0, 0, 0, 0, 0};
     const double4 pi4 = g.pos[i];
     T qi[5];
 #pragma unroll
-    for (int r = 0; r < 5; ++r) qi[r] = ri[kQOff + r];
+    for (int r = 0; r < 5; ++r) qi[r] = qa.q[5 * (long long)i + r];
     const T epsi2_i = T(limiter_epsi2(fp.vl_const, pi4.w));
     for (int j = 0; j < st.count; ++j) {
         const int k = st.list[j];
@@ -331,12 +320,14 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
         geo.xy = geo.xy + dels * delt_w;
         geo.yz = geo.yz + delt * deln_w;
         geo.zx = geo.zx + deln * dels_w;
-        T prim[5], Gi[5], Gk[5], rk[30];
-        reconstruct_end<T>(qi, ri + kDqOff, ri + kQmOff, epsi2_i, T(delx), T(dely), T(delz), prim);
+        T prim[5], Gi[5], Gk[5], qk[5];
+        reconstruct_end<T>(qi, qa.dq + 15 * (long long)i, qa.qm + 10 * (long long)i, epsi2_i, T(delx), T(dely),
+                           T(delz), prim);
         flux_eval<T>(st.family, st.axis, st.sign, ft, prim, T(fp.pi), Gi);
-        load_qrec<T>(recs + kQRec * (long long)k, rk);
-        reconstruct_end<T>(rk + kQOff, rk + kDqOff, rk + kQmOff, T(limiter_epsi2(fp.vl_const, pk.w)), T(delx),
-                           T(dely), T(delz), prim);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
+        reconstruct_end<T>(qk, qa.dq + 15 * (long long)k, qa.qm + 10 * (long long)k,
+                           T(limiter_epsi2(fp.vl_const, pk.w)), T(delx), T(dely), T(delz), prim);
         flux_eval<T>(st.family, st.axis, st.sign, ft, prim, T(fp.pi), Gk);
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
@@ -810,21 +801,13 @@ void launch_gather(double* dst, const double* src, const int* perm, int n, int m
     hipLaunchKernelGGL(k_gather, dim3(grid_for((long long)n * m, kBlock)), dim3(kBlock), 0, st, dst, src, perm,
                        n, m);
 }
-void launch_rec_put(const DevState& s, bool mixed, const double* src, int n, int m, int off, hipStream_t st) {
+void launch_to_float(float* dst, const double* src, long long n, hipStream_t st) {
     if (n <= 0) return;
-    const dim3 grid(grid_for((long long)n * m, kBlock));
-    if (mixed)
-        hipLaunchKernelGGL((k_rec_put<float>), grid, dim3(kBlock), 0, st, s.qrec32, src, n, m, off);
-    else
-        hipLaunchKernelGGL((k_rec_put<double>), grid, dim3(kBlock), 0, st, s.qrec, src, n, m, off);
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, dst, src, n);
 }
-void launch_rec_get(const DevState& s, bool mixed, double* dst, int n, int m, int off, hipStream_t st) {
+void launch_to_double(double* dst, const float* src, long long n, hipStream_t st) {
     if (n <= 0) return;
-    const dim3 grid(grid_for((long long)n * m, kBlock));
-    if (mixed)
-        hipLaunchKernelGGL((k_rec_get<float>), grid, dim3(kBlock), 0, st, dst, (const float*)s.qrec32, n, m, off);
-    else
-        hipLaunchKernelGGL((k_rec_get<double>), grid, dim3(kBlock), 0, st, dst, (const double*)s.qrec, n, m, off);
+    hipLaunchKernelGGL(k_to_double, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, dst, src, n);
 }
 void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st) {
     if (n <= 0) return;
@@ -876,8 +859,8 @@ void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams&
         else
             hipLaunchKernelGGL(k_qderiv_inner<false>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s, p,
                                temp);
-        hipLaunchKernelGGL((k_rec_put<double>), dim3(grid_for(15LL * g.n, kBlock)), dim3(kBlock), 0, st, s.qrec,
-                           (const double*)temp, g.n, 15, kDqOff);
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(15LL * g.n, kBlock)), dim3(kBlock), 0, st, s.dq,
+                           (const double*)temp, 15LL * g.n);
     }
 }
 void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {

@@ -15,8 +15,10 @@
 // and every wait ends.  A point publishes its delU by bumping its flag with
 // release semantics at agent scope; a workgroup first computes everything
 // that does not depend on this sweep's values (geometry, Cramer weights,
-// the old fluxes Gold), then spins with acquire loads on the flags of its
-// dependencies.  Every swept point is bumped exactly once per sweep, so when
+// the old fluxes Gold), then spins with relaxed agent-scope loads on the
+// flags of its dependencies and, once all are set, takes one acquire fence
+// before reading their values (an acquire load per poll would invalidate
+// the caches on every poll).  Every swept point is bumped exactly once per sweep, so when
 // a sweep starts all swept points carry equal flags and "finished in this
 // sweep" is flag[k] != flag[i].  A wait that exceeds the spin limit is
 // counted in the residual record and the host reports an error; the kernel
@@ -48,20 +50,17 @@ constexpr int kSweepWaves = kSweepBlock / 64;
 // about a second of polling; legitimate waits end within milliseconds
 constexpr int kSpinLimit = 1 << 20;
 
-__device__ __forceinline__ int flag_acquire(const int* f) {
-    return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ int flag_relaxed(const int* f) {
     return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// wait until point k has finished in this sweep; gives up after the spin
-// limit, or at once when another wait of this sweep already gave up, so a
-// broken schedule ends the kernel instead of stalling it point by point
+// wait until point k has finished in this sweep (the caller fences before
+// reading its values); gives up after the spin limit, or at once when
+// another wait of this sweep already gave up, so a broken schedule ends the
+// kernel instead of stalling it point by point
 __device__ __forceinline__ void wait_done(const int* fl, int my_flag, int* stalls) {
     int spins = 0;
-    while (flag_acquire(fl) == my_flag) {
+    while (flag_relaxed(fl) == my_flag) {
         ++spins;
         if ((spins & 255) == 0 && __hip_atomic_load(stalls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
             return;
@@ -150,6 +149,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
             if (base == 0) {
                 // wait for every dependency of this sweep position
                 for (long long q = db + tid; q < de; q += kSweepBlock) wait_done(a.flag + a.dep[q], my_flag, a.stalls);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 __syncthreads();
             }
             if (take) {

@@ -151,7 +151,7 @@ def bench_batch(args):
             w = offs[b] + np.flatnonzero(c.status == pc.STATUS_WALL)
             a = postprocess.wall_cp(f64["prim"][w], machs[b])
             m = postprocess.wall_cp(mx["prim"][w], machs[b])
-            errs.append(float(np.max(np.abs(m - a)) / np.max(np.abs(a))))
+            errs.append(float(np.max(np.abs(m - a))))
         vm = n * K / mx["el"] / 1e6
         mixed = {"value": round(vm, 3), "ms_per_step": round(mx["el"] / K * 1000, 4),
                  "speedup_vs_fp64": round(vm / value, 3), "wall_cp_err_vs_fp64_max": max(errs),
@@ -217,14 +217,16 @@ def main():
     log(f"bound; sweep schedule {s.sweep_info()}")
     tuning = None
     if not args.no_tune:
+        log("autotune")
         tuning = s.autotune()
         log(f"autotune: {tuning}")
 
     W, K = args.warmup, args.steps
+    log("warmup")
     if W > 0:
         s.run(1, W)
     s.synchronize()
-    log(f"warmup {W} done")
+    log(f"warmup {W} done; timed iterations")
     s.kernel_timing(True)
     t0 = time.perf_counter()
     rn, _ = s.run(W + 1, K)
@@ -262,11 +264,14 @@ def main():
         m = solver.Solver(cloud, cfg["mach"], cfg["aoa"], cfg["cfl"], mode=cfg["mode"],
                           precision="mixed", split=(cnt, lists), alias=alias)
         m.upload("prim", prim0)
+        log("mixed leg: autotune")
         if not args.no_tune:
             m.autotune()
+        log("mixed leg: warmup")
         if W > 0:
             m.run(1, W)
         m.synchronize()
+        log("mixed leg: timed iterations")
         t0 = time.perf_counter()
         rn_m, _ = m.run(W + 1, K)
         m.synchronize()
@@ -279,7 +284,11 @@ def main():
         v_m = cloud.n * K / el_m / 1e6
         mixed = {"value": round(v_m, 3), "ms_per_step": round(el_m / K * 1000.0, 4),
                  "speedup_vs_fp64": round(v_m / value, 3),
-                 "wall_cp_err_vs_fp64": float(np.max(np.abs(cpm - cp64)) / np.max(np.abs(cp64))),
+                 # Cp is already normalised by the dynamic pressure: the
+                 # absolute difference is the error; the relative one is
+                 # only meaningful once the wall flow has developed
+                 "wall_cp_err_vs_fp64": float(np.max(np.abs(cpm - cp64))),
+                 "wall_cp_err_rel_max_cp": float(np.max(np.abs(cpm - cp64)) / np.max(np.abs(cp64))),
                  "res_new_last": float(rn_m[-1]), "res_new_last_fp64": float(rn[-1])}
         log(f"mixed: {mixed}")
 

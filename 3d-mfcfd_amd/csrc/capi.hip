@@ -141,8 +141,8 @@ struct mfcfd_ctx {
     Schedule sched;
     std::vector<int> fwd_ptr, bwd_ptr;
     DevBuf<int> fwd_points, bwd_points;
-    // sync-free sweeps: points in sweep order, dependency lists, flags
-    DevBuf<int> fwd_seq, bwd_seq, fwd_dep, bwd_dep, sweep_flag;
+    // sync-free sweeps: points in sweep order, dependency lists, counters
+    DevBuf<int> fwd_seq, bwd_seq, fwd_dep, bwd_dep;
     DevBuf<long long> fwd_dep_ptr, bwd_dep_ptr;
     DevBuf<unsigned> sweep_ctr;
     // clouds of a batch (mfcfd_bind_batch): instance of each device point,
@@ -216,7 +216,7 @@ struct mfcfd_ctx {
         DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
                                &idx_fused_wall, &idx_generic, &partial_pos, &fwd_points, &bwd_points,
-                               &fwd_seq, &bwd_seq, &fwd_dep, &bwd_dep, &sweep_flag};
+                               &fwd_seq, &bwd_seq, &fwd_dep, &bwd_dep};
         for (auto* b : ints) b->release();
         fwd_dep_ptr.release();
         bwd_dep_ptr.release();
@@ -360,7 +360,6 @@ int do_sweeps(mfcfd_ctx* c) {
     }
     SweepAsync fwd, bwd;
     fwd.count = bwd.count = sch.n_swept;
-    fwd.flag = bwd.flag = c->sweep_flag.p;
     fwd.ctr = bwd.ctr = c->sweep_ctr.p;
     fwd.stalls = bwd.stalls = &c->rec.p->stalls;
     fwd.seq = c->fwd_seq.p;
@@ -795,39 +794,51 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     if ((rc = upload(c->fwd_points, fwd.data(), fwd.size(), st))) return rc;
     if ((rc = upload(c->bwd_points, bwd.data(), bwd.size(), st))) return rc;
     {
+        // execution order of the sync-free sweeps: the points sorted by
+        // dependency level (the level schedule's order).  Any order in which
+        // every point follows its dependencies gives the same results; this
+        // one keeps the resident window of tickets on a few adjacent levels,
+        // where the sweep order (alias order) can hold long dependency chains.
         const Schedule& S = c->sched;
         const size_t m = S.order.size();
-        std::vector<int> seq_f(m), seq_b(m), dep_f(S.fwd_dep.size()), dep_b(S.bwd_dep.size());
-        for (size_t t = 0; t < m; ++t) {
-            seq_f[t] = inv[S.order[t]];
-            seq_b[t] = inv[S.order[m - 1 - t]];
-        }
-        for (size_t e = 0; e < dep_f.size(); ++e) dep_f[e] = inv[S.fwd_dep[e]];
-        for (size_t e = 0; e < dep_b.size(); ++e) dep_b[e] = inv[S.bwd_dep[e]];
-        // every dependency of a sweep position must sit at an earlier
-        // position: the sync-free sweeps wait on nothing else
+        std::vector<int> seq_f(m), seq_b(m);
+        std::vector<long long> ptr_f(m + 1, 0), ptr_b(m + 1, 0);
+        std::vector<int> dep_f, dep_b;
+        dep_f.reserve(S.fwd_dep.size());
+        dep_b.reserve(S.bwd_dep.size());
+        // dependencies of each point: sweep position t of the forward sweep
+        // is order[t], of the backward sweep order[m-1-t]
         std::vector<int> at(n, -1);
         for (size_t t = 0; t < m; ++t) at[S.order[t]] = (int)t;
-        for (size_t t = 0; t < m; ++t) {
-            for (long long e = S.fwd_dep_ptr[t]; e < S.fwd_dep_ptr[t + 1]; ++e) {
-                const int q = at[S.fwd_dep[e]];
-                if (q < 0 || q >= (int)t) return fail(MFCFD_EINVAL, "forward sweep dependency out of order");
+        auto build = [&](const std::vector<int>& lev_points, const std::vector<long long>& dptr,
+                         const std::vector<int>& dep, bool forward, std::vector<int>& seq,
+                         std::vector<long long>& ptr, std::vector<int>& out) -> int {
+            std::vector<int> xpos(n, -1);
+            for (size_t u = 0; u < m; ++u) xpos[lev_points[u]] = (int)u;
+            for (size_t u = 0; u < m; ++u) {
+                const int p = lev_points[u];
+                seq[u] = inv[p];
+                const size_t t = forward ? (size_t)at[p] : m - 1 - (size_t)at[p];
+                for (long long e = dptr[t]; e < dptr[t + 1]; ++e) {
+                    // the sync-free sweeps wait on nothing but earlier positions
+                    if (xpos[dep[e]] < 0 || xpos[dep[e]] >= (int)u) return -1;
+                    out.push_back(inv[dep[e]]);
+                }
+                ptr[u + 1] = (long long)out.size();
             }
-            const size_t tb = m - 1 - t;   // backward position t is order[m-1-t]
-            for (long long e = S.bwd_dep_ptr[t]; e < S.bwd_dep_ptr[t + 1]; ++e) {
-                const int q = at[S.bwd_dep[e]];
-                if (q < 0 || q <= (int)tb) return fail(MFCFD_EINVAL, "backward sweep dependency out of order");
-            }
-        }
+            return 0;
+        };
+        if (build(S.fwd_points, S.fwd_dep_ptr, S.fwd_dep, true, seq_f, ptr_f, dep_f))
+            return fail(MFCFD_EINVAL, "forward sweep dependency out of order");
+        if (build(S.bwd_points, S.bwd_dep_ptr, S.bwd_dep, false, seq_b, ptr_b, dep_b))
+            return fail(MFCFD_EINVAL, "backward sweep dependency out of order");
         if ((rc = upload(c->fwd_seq, seq_f.data(), m, st))) return rc;
         if ((rc = upload(c->bwd_seq, seq_b.data(), m, st))) return rc;
         if ((rc = upload(c->fwd_dep, dep_f.data(), dep_f.size(), st))) return rc;
         if ((rc = upload(c->bwd_dep, dep_b.data(), dep_b.size(), st))) return rc;
-        if ((rc = upload(c->fwd_dep_ptr, S.fwd_dep_ptr.data(), S.fwd_dep_ptr.size(), st))) return rc;
-        if ((rc = upload(c->bwd_dep_ptr, S.bwd_dep_ptr.data(), S.bwd_dep_ptr.size(), st))) return rc;
-        if (c->sweep_flag.alloc((size_t)n) != hipSuccess || c->sweep_ctr.alloc(2) != hipSuccess)
-            return fail(MFCFD_ENOMEM, "sweep flags");
-        HIP_TRY(hipMemsetAsync(c->sweep_flag.p, 0, (size_t)n * sizeof(int), st));
+        if ((rc = upload(c->fwd_dep_ptr, ptr_f.data(), ptr_f.size(), st))) return rc;
+        if ((rc = upload(c->bwd_dep_ptr, ptr_b.data(), ptr_b.size(), st))) return rc;
+        if (c->sweep_ctr.alloc(2) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep counters");
         HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, 2 * sizeof(unsigned), st));
         HIP_TRY(hipMemsetAsync(c->rec.p, 0, sizeof(ResidualRecord), st));
         c->sweep_grid = sweep_async_grid((int)m, c->params.power == 0.0, false);

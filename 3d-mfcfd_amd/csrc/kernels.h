@@ -50,7 +50,6 @@ struct SweepAsync {
     const long long* dep_ptr;   // [count+1]: dependencies of position t are dep[dep_ptr[t] .. dep_ptr[t+1])
     const int* dep;             // device point numbers
     int count;
-    int* flag;                  // [n] per-point completion counter
     unsigned* ctr;              // [2] ticket counter, finished workgroups (zero between sweeps)
     int* stalls;                // waits that hit the spin limit
 };

@@ -9,20 +9,23 @@
 // graph is fixed by the alias order.
 //
 // Here a persistent grid of workgroups takes tickets from a counter; ticket
-// t processes position t of the sweep order.  Every point a workgroup waits
-// on has a smaller position, so it belongs to a workgroup that took its
-// ticket earlier and is running: the lowest unfinished ticket never waits
-// and every wait ends.  A point publishes its delU by bumping its flag with
-// release semantics at agent scope; a workgroup first computes everything
-// that does not depend on this sweep's values (geometry, Cramer weights,
-// the old fluxes Gold), then spins with relaxed agent-scope loads on the
-// flags of its dependencies and, once all are set, takes one acquire fence
-// before reading their values (an acquire load per poll would invalidate
-// the caches on every poll).  Every swept point is bumped exactly once per sweep, so when
-// a sweep starts all swept points carry equal flags and "finished in this
-// sweep" is flag[k] != flag[i].  A wait that exceeds the spin limit is
-// counted in the residual record and the host reports an error; the kernel
-// still runs to completion.
+// t processes position t of the execution order.  Every point a workgroup
+// waits on has a smaller position, so it belongs to a workgroup that took
+// its ticket earlier and is running: the lowest unfinished ticket never
+// waits and every wait ends.
+//
+// The values themselves are the completion flags.  Before a sweep, a kernel
+// marks the five outputs of every swept point with a signalling-NaN
+// pattern that no arithmetic produces (kPending).  A point writes its five
+// results once, with agent-scope relaxed atomic stores; a workgroup first
+// computes everything that does not depend on this sweep's values
+// (geometry, Cramer weights, the old fluxes Gold), then polls its
+// dependencies' values with agent-scope relaxed atomic loads until none is
+// pending, and reads them the same way.  Each 64-bit value is written once
+// and checked on its own, so no ordering between locations -- and no cache
+// write-back or invalidation -- is needed.  A wait that exceeds the spin
+// limit is counted in the residual record and the host reports an error;
+// the kernel still runs to completion.
 //
 // One workgroup of 128 lanes handles one point: lane e takes entry e of the
 // point's direction lists flattened in direction order (6 lists for an
@@ -50,17 +53,36 @@ constexpr int kSweepWaves = kSweepBlock / 64;
 // about a second of polling; legitimate waits end within milliseconds
 constexpr int kSpinLimit = 1 << 20;
 
-__device__ __forceinline__ int flag_relaxed(const int* f) {
-    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// a signalling NaN with a payload: arithmetic only ever produces quiet NaNs
+constexpr unsigned long long kPending = 0x7FF4DEAD5EEDF00DULL;
+
+__device__ __forceinline__ double load_value(const double* p) {
+    const unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)b);
 }
 
-// wait until point k has finished in this sweep (the caller fences before
-// reading its values); gives up after the spin limit, or at once when
-// another wait of this sweep already gave up, so a broken schedule ends the
-// kernel instead of stalling it point by point
-__device__ __forceinline__ void wait_done(const int* fl, int my_flag, int* stalls) {
+__device__ __forceinline__ void store_value(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool all_written(const double* v5) {
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+        ok = ok && __hip_atomic_load(reinterpret_cast<const unsigned long long*>(v5 + r), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) != kPending;
+    return ok;
+}
+
+// wait until all five values of a dependency are written in this sweep;
+// gives up after the spin limit, or at once when another wait of this sweep
+// already gave up, so a broken schedule ends the kernel instead of stalling
+// it point by point
+__device__ __forceinline__ void wait_done(const double* v5, int* stalls) {
     int spins = 0;
-    while (flag_relaxed(fl) == my_flag) {
+    while (!all_written(v5)) {
         ++spins;
         if ((spins & 255) == 0 && __hip_atomic_load(stalls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
             return;
@@ -109,7 +131,6 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         if (cls == 1) cnt[ZPOS] = 0;  // a wall point's z term walks conn (the ZNEG list)
 #pragma unroll
         for (int d = 0; d < 6; ++d) total += cnt[d];
-        const int my_flag = flag_relaxed(a.flag + i);
         const long long db = a.dep_ptr[t], de = a.dep_ptr[t + 1];
 
         double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -148,15 +169,15 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
             }
             if (base == 0) {
                 // wait for every dependency of this sweep position
-                for (long long q = db + tid; q < de; q += kSweepBlock) wait_done(a.flag + a.dep[q], my_flag, a.stalls);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                for (long long q = db + tid; q < de; q += kSweepBlock)
+                    wait_done(dU + 5 * (long long)a.dep[q], a.stalls);
                 __syncthreads();
             }
             if (take) {
                 double U[5], prim[5];
                 T pt[5], Gnew[5];
 #pragma unroll
-                for (int r = 0; r < 5; ++r) U[r] = dU[5 * (long long)k + r] + Uk[r];
+                for (int r = 0; r < 5; ++r) U[r] = load_value(dU + 5 * (long long)k + r) + Uk[r];
                 conserved_to_prim(U, fk, prim);
 #pragma unroll
                 for (int r = 0; r < 5; ++r) pt[r] = T(prim[r]);
@@ -167,7 +188,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         }
         if (total == 0) {
             // no entries: still honour the dependency wait order
-            for (long long q = db + tid; q < de; q += kSweepBlock) wait_done(a.flag + a.dep[q], my_flag, a.stalls);
+            for (long long q = db + tid; q < de; q += kSweepBlock) wait_done(dU + 5 * (long long)a.dep[q], a.stalls);
         }
 #pragma unroll
         for (int r = 0; r < 5; ++r) acc[r] = wave_sum(acc[r]);
@@ -189,13 +210,13 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
             if (FORWARD) {
 #pragma unroll
                 for (int r = 0; r < 5; ++r)
-                    s.delUp[5 * (long long)i + r] = -(D_inv * (s.flux_res[5 * (long long)i + r] + T5[r]));
+                    store_value(s.delUp + 5 * (long long)i + r,
+                                -(D_inv * (s.flux_res[5 * (long long)i + r] + T5[r])));
             } else {
 #pragma unroll
                 for (int r = 0; r < 5; ++r)
-                    s.delUn[5 * (long long)i + r] = s.delUp[5 * (long long)i + r] - D_inv * T5[r];
+                    store_value(s.delUn + 5 * (long long)i + r, s.delUp[5 * (long long)i + r] - D_inv * T5[r]);
             }
-            __hip_atomic_store(a.flag + i, my_flag + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
     }
@@ -207,6 +228,14 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
             __hip_atomic_store(a.ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+// mark the outputs of the swept points as not yet written
+__global__ void k_mark_pending(double* out, const int* seq, int count) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 5LL * count) return;
+    const long long t = e / 5;
+    reinterpret_cast<unsigned long long*>(out)[5LL * seq[t] + (e - 5 * t)] = kPending;
 }
 
 __global__ void k_copy_sweep(double* dst, const double* src, long long n) {
@@ -237,7 +266,10 @@ int pair_grid(int count) {
 template <bool UW, typename T>
 void launch_pair(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
                  const SweepAsync& bwd, int grid, hipStream_t st) {
+    const dim3 mg(grid_for(5LL * fwd.count, kBlock));
+    hipLaunchKernelGGL(k_mark_pending, mg, dim3(kBlock), 0, st, s.delUp, fwd.seq, fwd.count);
     hipLaunchKernelGGL((k_sweep_async<true, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, fwd);
+    hipLaunchKernelGGL(k_mark_pending, mg, dim3(kBlock), 0, st, s.delUn, bwd.seq, bwd.count);
     hipLaunchKernelGGL((k_sweep_async<false, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, bwd);
 }
 

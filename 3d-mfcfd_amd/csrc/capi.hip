@@ -1198,27 +1198,40 @@ int mfcfd_run_graph(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, dou
 
 namespace {
 
-// mean milliseconds of `repeats` back-to-back launches of fn (after one
-// untimed launch)
-template <typename F>
-int time_launches(mfcfd_ctx* c, int repeats, F fn, double* ms_out) {
-    hipEvent_t a, b;
-    HIP_TRY(hipEventCreate(&a));
-    if (hipEventCreate(&b) != hipSuccess) {
-        (void)hipEventDestroy(a);
-        return fail(MFCFD_EHIP, "hipEventCreate");
+// mean milliseconds of fn over `repeats` launches (after one untimed one),
+// each preceded by pre(): the kernel that runs before it in an iteration,
+// so every candidate meets the caches as the iteration leaves them
+template <typename P, typename F>
+int time_launches(mfcfd_ctx* c, int repeats, P pre, F fn, double* ms_out) {
+    std::vector<hipEvent_t> ev(2 * (size_t)repeats, nullptr);
+    int rc = 0;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            rc = fail(MFCFD_EHIP, "hipEventCreate");
+            break;
+        }
+    if (!rc) {
+        pre();
+        fn();
+        for (int r = 0; r < repeats; ++r) {
+            pre();
+            hipEventRecord(ev[2 * r], c->stream);
+            fn();
+            hipEventRecord(ev[2 * r + 1], c->stream);
+        }
+        rc = launch_error("autotune");
+        if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail(MFCFD_EHIP, "event synchronize");
     }
-    fn();
-    hipEventRecord(a, c->stream);
-    for (int r = 0; r < repeats; ++r) fn();
-    hipEventRecord(b, c->stream);
-    int rc = launch_error("autotune");
-    float ms = 0.f;
-    if (!rc && hipEventSynchronize(b) != hipSuccess) rc = fail(MFCFD_EHIP, "event synchronize");
-    if (!rc) hipEventElapsedTime(&ms, a, b);
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    *ms_out = ms / repeats;
+    double total = 0.0;
+    for (int r = 0; r < repeats && !rc; ++r) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]);
+        total += ms;
+    }
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    *ms_out = total / repeats;
     return rc;
 }
 
@@ -1233,9 +1246,33 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
     const DevGeom g = c->geom();
     const DevState s = c->state();
     const int blocks[3] = {64, 128, 256};
+    // LDS reservations that cap a kernel at the given waves per CU: fewer
+    // resident points per XCD keep their neighbours' data in that XCD's L2
+    int lds_cu = 0, lds_blk = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+        lds_cu = lds_blk = 0;
+    auto caps = [&](int block, std::initializer_list<int> waves_per_cu) {
+        std::vector<int> out;
+        for (int wcu : waves_per_cu) {
+            const int per_cu = wcu * 64 / block;   // blocks per CU
+            if (per_cu < 1 || per_cu * block != wcu * 64 || lds_cu <= 0) continue;
+            const int lds = (lds_cu / per_cu) & ~255;
+            if (lds <= 0 || lds > lds_blk || (per_cu + 1) * lds <= lds_cu) continue;
+            out.push_back(lds);
+        }
+        return out;
+    };
     mfcfd_tuning t{};
     // inputs of the timed kernels: the first steps of an iteration from prim
     launch_conserved_and_q(g, s, c->fp, c->stream);
+    // predecessors in the iteration: the q-variables before the q-derivatives,
+    // the q-derivatives (with the shape chosen for them) before the fluxes
+    auto pre_q = [&] { launch_q_variables(g, s, c->fp, c->stream); };
+    LaunchShapes best_for_pre;
+    auto pre_f = [&] {
+        if (c->fp.inner_iterations == 0) launch_qderiv_timestep(g, s, c->fp, c->stream, best_for_pre.qderiv);
+    };
     LaunchShapes best = c->shapes;
     if (c->fp.inner_iterations == 0) {
         const int waves[3] = {2, 4, 6};
@@ -1246,7 +1283,7 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
                 ks.block = bl;
                 ks.waves = w;
                 double ms = 0.0;
-                if ((rc = time_launches(c, repeats, [&] { launch_qderiv_timestep(g, s, c->fp, c->stream, ks); }, &ms)))
+                if ((rc = time_launches(c, repeats, pre_q, [&] { launch_qderiv_timestep(g, s, c->fp, c->stream, ks); }, &ms)))
                     return rc;
                 if (bl == 256 && w == 4) t.qderiv_default_ms = ms;
                 if (ms < best_ms) {
@@ -1254,7 +1291,19 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
                     best.qderiv = ks;
                 }
             }
+        for (int lds : caps(best.qderiv.block, {8, 12})) {
+            KernelShape ks = best.qderiv;
+            ks.lds = lds;
+            double ms = 0.0;
+            if ((rc = time_launches(c, repeats, pre_q, [&] { launch_qderiv_timestep(g, s, c->fp, c->stream, ks); }, &ms)))
+                return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.qderiv = ks;
+            }
+        }
         t.qderiv_ms = best_ms;
+        best_for_pre = best;
         launch_qderiv_timestep(g, s, c->fp, c->stream, best.qderiv);
     } else {
         launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
@@ -1272,7 +1321,7 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
                     LaunchShapes ls = best;
                     (part == 1 ? ls.flux_interior : ls.flux_wall) = KernelShape{bl, w};
                     double ms = 0.0;
-                    if ((rc = time_launches(c, repeats,
+                    if ((rc = time_launches(c, repeats, pre_f,
                                             [&] { launch_flux_residual(g, s, c->fp, c->tl, c->stream, part, ls); },
                                             &ms)))
                         return rc;
@@ -1282,6 +1331,20 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
                         pick = KernelShape{bl, w};
                     }
                 }
+            for (int lds : caps(pick.block, {4, 6})) {
+                LaunchShapes ls = best;
+                KernelShape ks = pick;
+                ks.lds = lds;
+                (part == 1 ? ls.flux_interior : ls.flux_wall) = ks;
+                double ms = 0.0;
+                if ((rc = time_launches(c, repeats, pre_f,
+                                        [&] { launch_flux_residual(g, s, c->fp, c->tl, c->stream, part, ls); }, &ms)))
+                    return rc;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    pick = ks;
+                }
+            }
             (part == 1 ? best.flux_interior : best.flux_wall) = pick;
             (part == 1 ? t.flux_interior_ms : t.flux_wall_ms) = best_ms;
         }
@@ -1297,7 +1360,7 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
         for (int dv : divs) {
             c->shapes.sweep_div = dv;
             double ms = 0.0;
-            if ((rc = time_launches(c, repeats, [&] { (void)do_sweeps(c); }, &ms))) return rc;
+            if ((rc = time_launches(c, repeats, [] {}, [&] { (void)do_sweeps(c); }, &ms))) return rc;
             if (dv == 1) t.sweeps_default_ms = ms;
             if (ms < best_ms) {
                 best_ms = ms;
@@ -1317,6 +1380,9 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
     t.qderiv_block = best.qderiv.block;
     t.qderiv_waves = best.qderiv.waves;
     t.sweep_div = best.sweep_div;
+    t.flux_interior_lds = best.flux_interior.lds;
+    t.flux_wall_lds = best.flux_wall.lds;
+    t.qderiv_lds = best.qderiv.lds;
     if (out) *out = t;
     return MFCFD_OK;
 }
@@ -1327,9 +1393,11 @@ int mfcfd_set_tuning(mfcfd_ctx* c, const mfcfd_tuning* in) {
     if (!ok_block(in->flux_interior_block) || !ok_block(in->flux_wall_block) || !ok_block(in->qderiv_block))
         return fail(MFCFD_EINVAL, "block sizes are 64, 128 or 256 (0: default)");
     if (in->sweep_div < 0 || in->sweep_div > 64) return fail(MFCFD_EINVAL, "sweep_div %d", in->sweep_div);
-    c->shapes.flux_interior = KernelShape{in->flux_interior_block, in->flux_interior_waves};
-    c->shapes.flux_wall = KernelShape{in->flux_wall_block, in->flux_wall_waves};
-    c->shapes.qderiv = KernelShape{in->qderiv_block, in->qderiv_waves};
+    if (in->flux_interior_lds < 0 || in->flux_wall_lds < 0 || in->qderiv_lds < 0)
+        return fail(MFCFD_EINVAL, "negative LDS reservation");
+    c->shapes.flux_interior = KernelShape{in->flux_interior_block, in->flux_interior_waves, in->flux_interior_lds};
+    c->shapes.flux_wall = KernelShape{in->flux_wall_block, in->flux_wall_waves, in->flux_wall_lds};
+    c->shapes.qderiv = KernelShape{in->qderiv_block, in->qderiv_waves, in->qderiv_lds};
     c->shapes.sweep_div = in->sweep_div > 0 ? in->sweep_div : 1;
     c->graph_stale = true;
     return MFCFD_OK;

@@ -28,6 +28,7 @@ struct TypeLists {
 struct KernelShape {
     int block = 0;
     int waves = 0;
+    int lds = 0;   // dynamic LDS bytes reserved per block: caps the blocks resident per CU
 };
 // shapes the autotuner chooses (defaults until it runs)
 struct LaunchShapes {

@@ -227,14 +227,14 @@ void launch_class(const DevGeom& g, const DevState& s, const FlowParams& p, cons
     const int block = ks.block > 0 && ks.block <= kBlock ? ks.block : kBlock;
     const dim3 grid(grid_for(count, block)), blk(block);
     if (p.power != 0.0) {
-        hipLaunchKernelGGL((k_flux_fused<CLASS, false, T, 2>), grid, blk, 0, st, g, s, p, idx, count);
+        hipLaunchKernelGGL((k_flux_fused<CLASS, false, T, 2>), grid, blk, ks.lds, st, g, s, p, idx, count);
         return;
     }
     switch (ks.waves) {
-        case 1: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 1>), grid, blk, 0, st, g, s, p, idx, count); break;
-        case 3: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 3>), grid, blk, 0, st, g, s, p, idx, count); break;
-        case 4: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 4>), grid, blk, 0, st, g, s, p, idx, count); break;
-        default: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 2>), grid, blk, 0, st, g, s, p, idx, count); break;
+        case 1: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 1>), grid, blk, ks.lds, st, g, s, p, idx, count); break;
+        case 3: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 3>), grid, blk, ks.lds, st, g, s, p, idx, count); break;
+        case 4: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 4>), grid, blk, ks.lds, st, g, s, p, idx, count); break;
+        default: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 2>), grid, blk, ks.lds, st, g, s, p, idx, count); break;
     }
 }
 

@@ -874,13 +874,13 @@ void launch_qdt(const DevGeom& g, const DevState& s, const FlowParams& p, hipStr
     const int block = ks.block > 0 && ks.block <= kBlock ? ks.block : kBlock;
     const dim3 grid(grid_for(g.n, block)), blk(block);
     if (p.power != 0.0) {
-        hipLaunchKernelGGL((k_qderiv_timestep<true, true, false, QT>), grid, blk, 0, st, g, s, p);
+        hipLaunchKernelGGL((k_qderiv_timestep<true, true, false, QT>), grid, blk, ks.lds, st, g, s, p);
         return;
     }
     switch (ks.waves) {
-        case 2: hipLaunchKernelGGL((k_qderiv_timestep<true, true, true, QT, 2>), grid, blk, 0, st, g, s, p); break;
-        case 6: hipLaunchKernelGGL((k_qderiv_timestep<true, true, true, QT, 6>), grid, blk, 0, st, g, s, p); break;
-        default: hipLaunchKernelGGL((k_qderiv_timestep<true, true, true, QT, 4>), grid, blk, 0, st, g, s, p); break;
+        case 2: hipLaunchKernelGGL((k_qderiv_timestep<true, true, true, QT, 2>), grid, blk, ks.lds, st, g, s, p); break;
+        case 6: hipLaunchKernelGGL((k_qderiv_timestep<true, true, true, QT, 6>), grid, blk, ks.lds, st, g, s, p); break;
+        default: hipLaunchKernelGGL((k_qderiv_timestep<true, true, true, QT, 4>), grid, blk, ks.lds, st, g, s, p); break;
     }
 }
 void launch_qderiv_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st,

@@ -132,6 +132,13 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
 #pragma unroll
         for (int d = 0; d < 6; ++d) total += cnt[d];
         const long long db = a.dep_ptr[t], de = a.dep_ptr[t + 1];
+        // the point's own inputs of the final update, read before the wait
+        double D_inv = 0.0, own[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (tid == 0) {
+            D_inv = s.dinv[i];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) own[r] = FORWARD ? s.flux_res[5 * (long long)i + r] : s.delUp[5 * (long long)i + r];
+        }
 
         double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int base = 0; base < total; base += kSweepBlock) {
@@ -206,16 +213,12 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 for (int w = 1; w < kSweepWaves; ++w) v = v + sh_part[w][r];
                 T5[r] = cls == 1 ? 2.0 * v : v;
             }
-            const double D_inv = s.dinv[i];
             if (FORWARD) {
 #pragma unroll
-                for (int r = 0; r < 5; ++r)
-                    store_value(s.delUp + 5 * (long long)i + r,
-                                -(D_inv * (s.flux_res[5 * (long long)i + r] + T5[r])));
+                for (int r = 0; r < 5; ++r) store_value(s.delUp + 5 * (long long)i + r, -(D_inv * (own[r] + T5[r])));
             } else {
 #pragma unroll
-                for (int r = 0; r < 5; ++r)
-                    store_value(s.delUn + 5 * (long long)i + r, s.delUp[5 * (long long)i + r] - D_inv * T5[r]);
+                for (int r = 0; r < 5; ++r) store_value(s.delUn + 5 * (long long)i + r, own[r] - D_inv * T5[r]);
             }
         }
         __syncthreads();

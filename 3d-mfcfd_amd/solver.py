@@ -49,6 +49,7 @@ class Tuning(C.Structure):
     _fields_ = [("flux_interior_block", C.c_int), ("flux_interior_waves", C.c_int),
                 ("flux_wall_block", C.c_int), ("flux_wall_waves", C.c_int),
                 ("qderiv_block", C.c_int), ("qderiv_waves", C.c_int), ("sweep_div", C.c_int),
+                ("flux_interior_lds", C.c_int), ("flux_wall_lds", C.c_int), ("qderiv_lds", C.c_int),
                 ("flux_interior_ms", C.c_double), ("flux_wall_ms", C.c_double),
                 ("qderiv_ms", C.c_double), ("sweeps_ms", C.c_double),
                 ("flux_interior_default_ms", C.c_double), ("flux_wall_default_ms", C.c_double),

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MFCFD_ABI_VERSION 3
+#define MFCFD_ABI_VERSION 4
 
 /* error codes */
 #define MFCFD_OK 0
@@ -201,8 +201,10 @@ int mfcfd_run_graph(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, d
  * bound cloud(s) and the current state, every compiled variant of the
  * gather kernels -- block size 64/128/256 x minimum waves per SIMD (the
  * register cap: 2/4/6 for the q-derivative kernel, 1..4 for the fused flux
- * kernels) -- and, in implicit mode, the sync-free sweeps on 1, 1/2 or 1/4
- * of the resident grid; keeps the fastest.  Results do not depend on the
+ * kernels), then for the chosen shape an LDS reservation that caps the
+ * resident waves per CU (fewer points in flight per XCD keep their
+ * neighbours in that XCD's L2) -- and, in implicit mode, the sync-free
+ * sweeps on 1, 1/2 or 1/4 of the resident grid; keeps the fastest.  Results do not depend on the
  * shapes.  It recomputes U, q, qm, dq, delt, flux_res and (implicit)
  * delUp/delUn from prim as one iteration's first steps would; prim is not
  * touched. */
@@ -211,6 +213,7 @@ typedef struct mfcfd_tuning {
     int flux_wall_block, flux_wall_waves;
     int qderiv_block, qderiv_waves;
     int sweep_div;
+    int flux_interior_lds, flux_wall_lds, qderiv_lds;   /* dynamic LDS bytes per block (0: none) */
     double flux_interior_ms, flux_wall_ms, qderiv_ms, sweeps_ms;   /* chosen shapes */
     double flux_interior_default_ms, flux_wall_default_ms, qderiv_default_ms, sweeps_default_ms;
 } mfcfd_tuning;

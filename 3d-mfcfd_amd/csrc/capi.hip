@@ -1331,7 +1331,7 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
                         pick = KernelShape{bl, w};
                     }
                 }
-            for (int lds : caps(pick.block, {4, 6})) {
+            for (int lds : caps(pick.block, {4, 6, 8, 12})) {
                 LaunchShapes ls = best;
                 KernelShape ks = pick;
                 ks.lds = lds;

@@ -76,7 +76,11 @@ __device__ __forceinline__ void reconstruct_end(const T q[5], const T* dq, const
     }
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
-        const T phi = venkat_phi(qt[r], q[r], qm[2 * r], qm[2 * r + 1], epsi2);
+        // FP64: the reference's qtilde - q.  FP32: the same quantity as
+        // -0.5*temp; qtilde - q cancels catastrophically against the
+        // limiter's 1e-5 switch in single precision
+        const T del_neg = sizeof(T) == 8 ? qt[r] - q[r] : T(-0.5) * temp[r];
+        const T phi = venkat_phi_dn(del_neg, q[r], qm[2 * r], qm[2 * r + 1], epsi2);
         qt[r] = q[r] - T(0.5) * phi * temp[r];
     }
     qtilde_to_primitive(qt, prim);

@@ -166,9 +166,25 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
     const Frame<T> ft = frame_as<T>(f);
     const T pi = T(fp.pi);
     const double4 pi4 = g.pos[i];
+    // the point's own reconstruction inputs: FP32 keeps them in registers
+    // (re-reading them for every neighbour doubles the requests to L2); the
+    // FP64 kernel has no registers to spare and re-reads them
+    constexpr bool kOwnInRegs = sizeof(T) == 4;
+    T own[kOwnInRegs ? 30 : 1];
     const T* qip = qa.q + 5 * (long long)i;
     const T* dqi = qa.dq + 15 * (long long)i;
     const T* qmi = qa.qm + 10 * (long long)i;
+    if constexpr (kOwnInRegs) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) own[r] = qip[r];
+#pragma unroll
+        for (int r = 0; r < 15; ++r) own[5 + r] = dqi[r];
+#pragma unroll
+        for (int r = 0; r < 10; ++r) own[20 + r] = qmi[r];
+        qip = own;
+        dqi = own + 5;
+        qmi = own + 20;
+    }
     const T epsi2_i = T(limiter_epsi2(fp.vl_const, pi4.w));
     const double* cf = g.coef + 18 * (long long)i;
     const int nb = g.nbhs[i];

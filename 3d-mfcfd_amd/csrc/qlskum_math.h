@@ -141,10 +141,9 @@ __device__ __forceinline__ void flux_eval(int family, int axis, int sign, const 
         flux_octant(axis, sign, family == 2 ? 1 : -1, f, prim, pi, G);
 }
 
-// limiters.f90:18-40 for one component
+// limiters.f90:18-40 for one component, from del_neg = qtilde - q
 template <typename T>
-__device__ __forceinline__ T venkat_phi(T qt, T q, T qmax, T qmin, T epsi2) {
-    const T del_neg = qt - q;
+__device__ __forceinline__ T venkat_phi_dn(T del_neg, T q, T qmax, T qmin, T epsi2) {
     const T thr = T(kLimiterThreshold);
     if (dabs(del_neg) <= thr) return T(1);
     if (!(dabs(del_neg) > thr)) return T(1);  // NaN: the reference leaves phi unset
@@ -156,6 +155,11 @@ __device__ __forceinline__ T venkat_phi(T qt, T q, T qmax, T qmin, T epsi2) {
     den = den * del_neg;
     const T temp = num / den;
     return temp < T(1) ? temp : T(1);
+}
+
+template <typename T>
+__device__ __forceinline__ T venkat_phi(T qt, T q, T qmax, T qmin, T epsi2) {
+    return venkat_phi_dn(qt - q, q, qmax, qmin, epsi2);
 }
 
 // qvariables_to_primitive_variables.f90:16-30

@@ -189,6 +189,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--phase-iters", type=int, default=3)
+    ap.add_argument("--cp-iters", type=int, default=0,
+                    help="iterations at which FP64 and mixed wall Cp are compared (default: the "
+                         "configuration's own count for explicit runs, 60 for implicit ones)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -241,7 +244,6 @@ def main():
 
     phases = s.time_phases(W + K + 1, args.phase_iters)
     log("phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
-    t_next = W + K + args.phase_iters + 1
     by = roofline.iteration_bytes(cloud.status, cnt, cloud.nbhs, cfg["mode"])
     cls = s.point_class_map()
     flux_bytes = int(roofline.flux_bytes_per_point(cloud.status, cnt, cloud.nbhs)[cls == 0].sum())
@@ -257,6 +259,16 @@ def main():
             "launches_timed": flux_launches}
     step_gbs = by["total"] / (ms_per_step * 1e-3) / 1e9
 
+    # accuracy: both forms from the same start to the same iteration count
+    n_done = W + K + args.phase_iters
+    n_cp = args.cp_iters if args.cp_iters > 0 else (cfg["iters"] if cfg["mode"] == "explicit" else 60)
+    n_cp = max(n_cp, n_done)
+    if not args.no_mixed and n_cp > n_done:
+        log(f"FP64 to iteration {n_cp} for the wall-Cp comparison")
+        s.run(n_done + 1, n_cp - n_done)
+    else:
+        n_cp = n_done
+    t_next = n_cp + 1
     prim_fp64 = s.download("prim")
     mixed = None
     if not args.no_mixed:
@@ -276,6 +288,10 @@ def main():
         rn_m, _ = m.run(W + 1, K)
         m.synchronize()
         el_m = time.perf_counter() - t0
+        phases_m = m.time_phases(W + K + 1, args.phase_iters) if args.phase_iters > 0 else None
+        if n_cp > n_done:
+            log(f"mixed to iteration {n_cp} for the wall-Cp comparison")
+            m.run(n_done + 1, n_cp - n_done)
         prim_m = m.download("prim")
         m.close()
         wall = cloud.status == pc.STATUS_WALL
@@ -288,8 +304,10 @@ def main():
                  # absolute difference is the error; the relative one is
                  # only meaningful once the wall flow has developed
                  "wall_cp_err_vs_fp64": float(np.max(np.abs(cpm - cp64))),
+                 "wall_cp_iterations": n_cp,
                  "wall_cp_err_rel_max_cp": float(np.max(np.abs(cpm - cp64)) / np.max(np.abs(cp64))),
-                 "res_new_last": float(rn_m[-1]), "res_new_last_fp64": float(rn[-1])}
+                 "res_new_last": float(rn_m[-1]), "res_new_last_fp64": float(rn[-1]),
+                 "phases_ms": {k: round(v, 4) for k, v in phases_m.items()} if phases_m else None}
         log(f"mixed: {mixed}")
 
     cpu = None

@@ -556,6 +556,8 @@ namespace {
 int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_inst, const double* fs) {
     int rc = 0;
     if (!gm || gm->n <= 0) return fail(MFCFD_EINVAL, "empty geometry");
+    // gather kernels run 4 lanes per point; thread indices are 32-bit
+    if (gm->n > (1 << 28)) return fail(MFCFD_EINVAL, "%d points: at most 2^28 per context", gm->n);
     const int n = gm->n;
     const double* dp[] = {gm->x, gm->y, gm->z, gm->tan1, gm->tan2, gm->nor, gm->min_dist};
     for (auto* p : dp)

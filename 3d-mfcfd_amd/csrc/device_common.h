@@ -30,6 +30,37 @@ __device__ __forceinline__ int xcd_block() {
 // index of this thread when the launch covers count items, blocks renumbered per XCD
 __device__ __forceinline__ int xcd_index() { return xcd_block() * (int)blockDim.x + (int)threadIdx.x; }
 
+// Sum of v over the G consecutive lanes of an aligned lane group (G a power
+// of two).  Every lane of the group forms the same pairwise tree, so every
+// lane receives the same, order-determined value.
+template <int G, typename V>
+__device__ __forceinline__ V group_sum(V v) {
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+
+// group maximum / minimum with the comparisons of a sequential scan
+// (a NaN operand never replaces a value)
+template <int G, typename V>
+__device__ __forceinline__ V group_max(V v) {
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        const V w = __shfl_xor(v, o, G);
+        if (w > v) v = w;
+    }
+    return v;
+}
+template <int G, typename V>
+__device__ __forceinline__ V group_min(V v) {
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        const V w = __shfl_xor(v, o, G);
+        if (w < v) v = w;
+    }
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ Frame<T> frame_as(const Frame<double>& f) {
     Frame<T> o;

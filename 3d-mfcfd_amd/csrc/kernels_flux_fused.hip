@@ -14,6 +14,14 @@
 // applied edge by edge, through the Cramer weights of each list
 // (sum_j (G_k - G_i)*(c . (dels, delt, deln)) instead of the 3x3 solve of the
 // accumulated sums), which changes the rounding, not the mathematics.
+//
+// A point is handled by a group of kFluxGroup lanes, each taking every
+// kFluxGroup-th neighbour, and the group sums its partial residuals at the
+// end.  Per lane this is the same work as one lane per point, but a resident
+// wave holds 64 / kFluxGroup points instead of 64: the neighbour data of all
+// points in flight on one XCD then fits its 4 MB L2.  With one lane per point
+// at the 8 resident waves per CU the FP64 kernel's registers allow, each
+// point's q, qm and dq were fetched from HBM about 19 times per launch.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -26,6 +34,8 @@
 namespace mfcfd {
 
 namespace {
+
+constexpr int kFluxGroup = 4;
 
 // quantities of one reconstructed end state shared by every flux of the edge
 template <typename T>
@@ -158,8 +168,10 @@ __device__ __forceinline__ void wall_axis(const EndState<T>& ei, const EndState<
 template <int CLASS, bool UW, typename T, int WAVES>
 __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevState s, FlowParams fp, const int* idx,
                                                              int count) {
-    const int m = xcd_index();
-    if (m >= count) return;
+    const unsigned t = (unsigned)xcd_index();
+    const int m = (int)(t / kFluxGroup);
+    const int lane = (int)(t % kFluxGroup);
+    if (m >= count) return;  // whole groups: blocks hold a multiple of kFluxGroup lanes
     const QArrays<T> qa = qarrays<T>(s);
     const int i = idx[m];
     const Frame<double> f = load_frame(g.frame, i);
@@ -191,7 +203,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
     const int* cn = g.conn + kConnCols * (long long)i;
     const unsigned char* mk = g.mask + kConnCols * (long long)i;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int j = 0; j < nb; ++j) {
+    for (int j = lane; j < nb; j += kFluxGroup) {
         const unsigned bits = mk[j];
         if (bits == 0u) continue;
         const int k = cn[j];
@@ -233,15 +245,20 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
         }
     }
 #pragma unroll
-    for (int r = 0; r < 5; ++r) s.flux_res[5 * (long long)i + r] = CLASS == 1 ? 2.0 * acc[r] : acc[r];
+    for (int r = 0; r < 5; ++r) acc[r] = group_sum<kFluxGroup>(acc[r]);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) s.flux_res[5 * (long long)i + r] = CLASS == 1 ? 2.0 * acc[r] : acc[r];
+    }
 }
 
 template <int CLASS, typename T>
 void launch_class(const DevGeom& g, const DevState& s, const FlowParams& p, const int* idx, int count,
                   const KernelShape& ks, hipStream_t st) {
     if (count <= 0) return;
-    const int block = ks.block > 0 && ks.block <= kBlock ? ks.block : kBlock;
-    const dim3 grid(grid_for(count, block)), blk(block);
+    int block = ks.block > 0 && ks.block <= kBlock ? ks.block : kBlock;
+    block -= block % kFluxGroup;
+    const dim3 grid(grid_for((long long)count * kFluxGroup, block)), blk(block);
     if (p.power != 0.0) {
         hipLaunchKernelGGL((k_flux_fused<CLASS, false, T, 2>), grid, blk, ks.lds, st, g, s, p, idx, count);
         return;

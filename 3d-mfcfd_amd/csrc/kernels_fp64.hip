@@ -128,10 +128,17 @@ __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
 // FP64 form; float: the mixed form); the geometric sums and the 3x3 solve
 // stay FP64
 // WAVES: minimum waves per SIMD of the compiled variant (register cap)
+// A point is handled by kQdGroup lanes, each taking every kQdGroup-th
+// neighbour; the group combines its partial sums, maxima and minima at the
+// end.  Fewer points in flight per XCD keep their neighbours' q, prim and
+// positions resident in its L2.
+constexpr int kQdGroup = 4;
 template <bool DO_DQ, bool DO_DT, bool UW, typename QT, int WAVES = 4>
 __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, DevState s, FlowParams fp) {
-    const int i = xcd_index();
-    if (i >= g.n) return;
+    const unsigned t = (unsigned)xcd_index();
+    const int i = (int)(t / kQdGroup);
+    const int lane = (int)(t % kQdGroup);
+    if (i >= g.n) return;  // whole groups: blocks hold a multiple of kQdGroup lanes
     const QArrays<QT> qa = qarrays<QT>(s);
     const double4 pi4 = g.pos[i];
     const int nb = g.nbhs[i];
@@ -149,7 +156,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
         }
     }
     double min_delt = 1.0;
-    for (int j = 0; j < nb; ++j) {
+    for (int j = lane; j < nb; j += kQdGroup) {
         const int k = cn[j];
         const double4 pk = g.pos[k];
         const double delx = pk.x - pi4.x;
@@ -189,6 +196,24 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
             if (delt < min_delt) min_delt = delt;
         }
     }
+    if (DO_DQ) {
+        geo.xx = group_sum<kQdGroup>(geo.xx);
+        geo.yy = group_sum<kQdGroup>(geo.yy);
+        geo.zz = group_sum<kQdGroup>(geo.zz);
+        geo.xy = group_sum<kQdGroup>(geo.xy);
+        geo.yz = group_sum<kQdGroup>(geo.yz);
+        geo.zx = group_sum<kQdGroup>(geo.zx);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            sx[r] = group_sum<kQdGroup>(sx[r]);
+            sy[r] = group_sum<kQdGroup>(sy[r]);
+            sz[r] = group_sum<kQdGroup>(sz[r]);
+            qmax[r] = group_max<kQdGroup>(qmax[r]);
+            qmin[r] = group_min<kQdGroup>(qmin[r]);
+        }
+    }
+    if (DO_DT) min_delt = group_min<kQdGroup>(min_delt);
+    if (lane != 0) return;
     if (DO_DQ) {
         const double det = cramer_det(geo);
         const double one_by_det = 1.0 / det;
@@ -840,7 +865,7 @@ void launch_conserved_and_q(const DevGeom& g, const DevState& s, const FlowParam
 void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams& p, double* temp,
                           hipStream_t st) {
     if (g.n <= 0) return;
-    const dim3 grid(grid_for(g.n, kBlock));
+    const dim3 grid(grid_for((long long)g.n * kQdGroup, kBlock));
     if (p.mixed) {
         if (p.power == 0.0)
             hipLaunchKernelGGL((k_qderiv_timestep<true, false, true, float>), grid, dim3(kBlock), 0, st, g, s, p);
@@ -866,13 +891,14 @@ void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams&
 void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
     if (g.n <= 0) return;
     // no least-squares weights in the time step
-    hipLaunchKernelGGL((k_qderiv_timestep<false, true, true, double>), dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0,
-                       st, g, s, p);
+    hipLaunchKernelGGL((k_qderiv_timestep<false, true, true, double>), dim3(grid_for((long long)g.n * kQdGroup, kBlock)),
+                       dim3(kBlock), 0, st, g, s, p);
 }
 template <typename QT>
 void launch_qdt(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st, const KernelShape& ks) {
-    const int block = ks.block > 0 && ks.block <= kBlock ? ks.block : kBlock;
-    const dim3 grid(grid_for(g.n, block)), blk(block);
+    int block = ks.block > 0 && ks.block <= kBlock ? ks.block : kBlock;
+    block -= block % kQdGroup;
+    const dim3 grid(grid_for((long long)g.n * kQdGroup, block)), blk(block);
     if (p.power != 0.0) {
         hipLaunchKernelGGL((k_qderiv_timestep<true, true, false, QT>), grid, blk, ks.lds, st, g, s, p);
         return;

@@ -191,7 +191,7 @@ def main():
     ap.add_argument("--phase-iters", type=int, default=3)
     ap.add_argument("--cp-iters", type=int, default=0,
                     help="iterations at which FP64 and mixed wall Cp are compared (default: the "
-                         "configuration's own count for explicit runs, 60 for implicit ones)")
+                         "configuration's own count for explicit runs, 10 for implicit ones)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -261,7 +261,12 @@ def main():
 
     # accuracy: both forms from the same start to the same iteration count
     n_done = W + K + args.phase_iters
-    n_cp = args.cp_iters if args.cp_iters > 0 else (cfg["iters"] if cfg["mode"] == "explicit" else 60)
+    # Implicit runs compare at iteration 10: the reference updates outer
+    # points explicitly (U = Ubar - dt*R) with the implicit run's CFL-50 local
+    # step, and that update amplifies a far-field disturbance about 1.8x per
+    # iteration, in FP64 and in the oracle alike; on the 8M cloud the
+    # pressures turn non-finite from iteration 14.
+    n_cp = args.cp_iters if args.cp_iters > 0 else (cfg["iters"] if cfg["mode"] == "explicit" else 10)
     n_cp = max(n_cp, n_done)
     if not args.no_mixed and n_cp > n_done:
         log(f"FP64 to iteration {n_cp} for the wall-Cp comparison")
@@ -277,8 +282,7 @@ def main():
                           precision="mixed", split=(cnt, lists), alias=alias)
         m.upload("prim", prim0)
         log("mixed leg: autotune")
-        if not args.no_tune:
-            m.autotune()
+        tuning_m = m.autotune() if not args.no_tune else None
         log("mixed leg: warmup")
         if W > 0:
             m.run(1, W)
@@ -305,9 +309,12 @@ def main():
                  # only meaningful once the wall flow has developed
                  "wall_cp_err_vs_fp64": float(np.max(np.abs(cpm - cp64))),
                  "wall_cp_iterations": n_cp,
+                 "nonfinite_points": [int((~np.isfinite(prim_fp64)).any(1).sum()),
+                                      int((~np.isfinite(prim_m)).any(1).sum())],
                  "wall_cp_err_rel_max_cp": float(np.max(np.abs(cpm - cp64)) / np.max(np.abs(cp64))),
                  "res_new_last": float(rn_m[-1]), "res_new_last_fp64": float(rn[-1]),
-                 "phases_ms": {k: round(v, 4) for k, v in phases_m.items()} if phases_m else None}
+                 "phases_ms": {k: round(v, 4) for k, v in phases_m.items()} if phases_m else None,
+                 "tuning": tuning_m}
         log(f"mixed: {mixed}")
 
     cpu = None

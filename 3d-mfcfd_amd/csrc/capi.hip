@@ -121,7 +121,7 @@ struct mfcfd_ctx {
     // geometry, device order
     DevBuf<double4> pos;
     DevBuf<double> frame, coef;
-    DevBuf<int> status, nbhs, conn, dir_cnt, lst, alias, respos, perm;
+    DevBuf<int> status, nbhs, conn, dir_cnt, lst, alias, respos, perm, walk, nwalk;
     DevBuf<long long> dir_off;
     DevBuf<unsigned char> mask;
     DevBuf<int> idx_interior, idx_wall, idx_outer, idx_sup_in, idx_sup_out;
@@ -164,7 +164,7 @@ struct mfcfd_ctx {
     double res_old = 0.0, res_new = 0.0, residue = 0.0, max_res = 0.0, sum_res_sqr = 0.0;
     int max_res_point = 0, last_t = 0;
     int n_regular = 0;
-    std::vector<int> host_class;   // per host point: 0 fused interior, 1 fused wall, 2 list-by-list
+    std::vector<int> host_class;   // per host point: 0 fused interior, 1 fused general, 2 list-by-list
     // HIP events around each fused interior flux launch (mfcfd_kernel_timing)
     bool time_flux = false;
     std::vector<hipEvent_t> tev;
@@ -183,6 +183,8 @@ struct mfcfd_ctx {
         g.dir_cnt = dir_cnt.p;
         g.dir_off = dir_off.p;
         g.lst = lst.p;
+        g.walk = walk.p;
+        g.nwalk = nwalk.p;
         g.mask = mask.p;
         g.coef = coef.p;
         g.alias = alias.p;
@@ -213,7 +215,8 @@ struct mfcfd_ctx {
         DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &q, &qm, &dq, &delt, &flux_res, &delUp, &delUn,
                                  &dinv, &temp, &staging, &partial, &hist};
         for (auto* b : dbl) b->release();
-        DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &idx_interior,
+        DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &walk, &nwalk,
+                               &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
                                &idx_fused_wall, &idx_generic, &partial_pos, &fwd_points, &bwd_points,
                                &fwd_seq, &bwd_seq, &fwd_dep, &bwd_dep};
@@ -627,58 +630,73 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     std::vector<long long> off(6ULL * n, 0);
     std::vector<int> lst;
     lst.reserve((size_t)n * 76);
-    std::vector<unsigned char> mask(27ULL * n, 0);
-    std::vector<unsigned char> regular(n, 0);
+    // The fused flux kernel visits each point's walk once: conn, with a bit
+    // per direction list an entry belongs to, then every list entry that does
+    // not continue an in-order match against conn (the entries the 22-column
+    // spill carries over from the neighbouring point), each with its own
+    // list's bit.  Every list is then exactly the walk entries carrying its
+    // bit, duplicates included.  A point whose walk would exceed kWalkCols
+    // entries takes the list-by-list kernel.  Interior points whose walk is
+    // conn with each entry in exactly one list per axis take the fused
+    // kernel's interior class; the others, with the wall and outer points,
+    // its general class.
+    std::vector<int> walk((size_t)kWalkCols * n, 0), nwalk(n, 0);
+    std::vector<unsigned char> mask((size_t)kWalkCols * n, 0);
+    std::vector<unsigned char> walkable(n, 0);
+    c->n_regular = 0;
     for (int d = 0; d < n; ++d) {
         const int r = perm[d];
         const int st = status[d];
-        bool reg = (st == 0 || st == 1);
+        bool fits = (st == 0 || st == 1 || st == 2);
+        bool reg = true;
         const int nb = nbhs[d];
         const int* cr = gm->conn + 27LL * r;
+        int* wk = walk.data() + (size_t)kWalkCols * d;
+        unsigned char* mk = mask.data() + (size_t)kWalkCols * d;
+        for (int j = 0; j < nb; ++j) wk[j] = conn0[27ULL * d + j];
+        int nw = nb;
         for (int dir = 0; dir < 6; ++dir) {
             const HostList hl = host_list(gm, r, dir);
             off[(size_t)dir * n + d] = (long long)lst.size();
             cnt[(size_t)dir * n + d] = hl.p ? hl.count : 0;
             if (!hl.p) continue;
-            // membership: an in-order subsequence of conn sets mask bits
             int j = 0;
             for (int e = 0; e < hl.count; ++e) {
                 const int k1 = hl.p[e];
                 lst.push_back(inv[k1 - 1]);
                 while (j < nb && cr[j] != k1) ++j;
                 if (j < nb) {
-                    mask[27ULL * d + j] |= (unsigned char)(1u << dir);
+                    mk[j] |= (unsigned char)(1u << dir);
                     ++j;
+                } else if (nw < kWalkCols) {
+                    reg = false;
+                    wk[nw] = inv[k1 - 1];
+                    mk[nw] = (unsigned char)(1u << dir);
+                    ++nw;
                 } else {
                     reg = false;
+                    fits = false;
                 }
             }
         }
-        if (reg) {
-            // interior: each axis partitions conn; wall: x/y cover conn and
-            // zneg is conn itself
-            for (int j = 0; j < nb && reg; ++j) {
-                const unsigned b = mask[27ULL * d + j];
-                for (int a = 0; a < 3; ++a) {
-                    const unsigned ab = (b >> (2 * a)) & 3u;
-                    if (st == 0 && !(ab == 1u || ab == 2u)) reg = false;
-                    if (st == 1 && a < 2 && ab == 0u) reg = false;
-                }
-                if (st == 1 && !(b & (1u << ZNEG))) reg = false;
+        bool simple = st == 0 && reg;
+        for (int j = 0; j < nb && simple; ++j)
+            for (int a = 0; a < 3; ++a) {
+                const unsigned ab = (mk[j] >> (2 * a)) & 3u;
+                if (!(ab == 1u || ab == 2u)) simple = false;
             }
-        }
-        regular[d] = reg ? 1 : 0;
+        nwalk[d] = nw;
+        walkable[d] = fits ? (simple ? 2 : 1) : 0;
+        c->n_regular += reg ? 1 : 0;
     }
-    c->n_regular = 0;
-    for (int d = 0; d < n; ++d) c->n_regular += regular[d];
 
     // class lists (device numbers, device order) and the residual order
     std::vector<int> li, lw, lo, lsi, lso, lfi, lfw, lgen;
     for (int d = 0; d < n; ++d) {
         switch (status[d]) {
-            case 0: li.push_back(d); (regular[d] ? lfi : lgen).push_back(d); break;
-            case 1: lw.push_back(d); (regular[d] ? lfw : lgen).push_back(d); break;
-            case 2: lo.push_back(d); lgen.push_back(d); break;
+            case 0: li.push_back(d); (walkable[d] == 2 ? lfi : walkable[d] ? lfw : lgen).push_back(d); break;
+            case 1: lw.push_back(d); (walkable[d] ? lfw : lgen).push_back(d); break;
+            case 2: lo.push_back(d); (walkable[d] ? lfw : lgen).push_back(d); break;
             case 5: lsi.push_back(d); break;
             case 6: lso.push_back(d); break;
         }
@@ -710,6 +728,8 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     if ((rc = upload(c->dir_off, off.data(), off.size(), st))) return rc;
     if ((rc = upload(c->lst, lst.data(), lst.size(), st))) return rc;
     if ((rc = upload(c->mask, mask.data(), mask.size(), st))) return rc;
+    if ((rc = upload(c->walk, walk.data(), walk.size(), st))) return rc;
+    if ((rc = upload(c->nwalk, nwalk.data(), nwalk.size(), st))) return rc;
     if ((rc = upload(c->alias, alias.data(), alias.size(), st))) return rc;
     if ((rc = upload(c->respos, respos.data(), respos.size(), st))) return rc;
     if ((rc = upload(c->perm, perm.data(), perm.size(), st))) return rc;

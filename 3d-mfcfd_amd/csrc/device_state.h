@@ -32,6 +32,9 @@
 namespace mfcfd {
 
 constexpr int kConnCols = 27;
+// entries per point of the fused flux kernel's walk: conn, then the list
+// entries that are not an in-order subsequence of conn (22-column spill)
+constexpr int kWalkCols = 32;
 constexpr int kSplitCols = 22;
 constexpr int kSplitSlack = 32;
 
@@ -47,7 +50,9 @@ struct DevGeom {
     const int* dir_cnt;          // [6][n]
     const long long* dir_off;    // [6][n]
     const int* lst;
-    const unsigned char* mask;   // [27n]
+    const int* walk;             // [kWalkCols n] entries the fused flux kernel visits
+    const int* nwalk;            // [n] used entries of walk
+    const unsigned char* mask;   // [kWalkCols n] direction lists each walk entry belongs to
     const double* coef;          // [18n]
     const int* alias;
     const int* respos;           // position in the residual order (wall, then interior)

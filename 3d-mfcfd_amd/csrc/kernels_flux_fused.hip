@@ -1,12 +1,13 @@
-// kernels_flux_fused.hip -- flux residual of regular interior and wall points,
-// one pass over the neighbour list, in FP64 or in the mixed form.
+// kernels_flux_fused.hip -- flux residual of interior, wall and outer points,
+// one pass over each point's walk, in FP64 or in the mixed form.
 //
 // The reference walks each of a point's direction lists separately and
 // recomputes, for every list an edge (i,k) belongs to, the same limited
 // reconstruction at both ends (interior_flux_dGxpos.f90:92-104).  An
 // interior neighbour belongs to exactly one list per axis, so each edge is
-// reconstructed three times.  For a regular point (every list an in-order
-// subsequence of conn) this kernel walks conn once, reconstructs each end
+// reconstructed three times.  This kernel walks each point's walk once (conn
+// with a membership bit per direction list, then the list entries that are
+// not an in-order subsequence of conn), reconstructs each end
 // once, evaluates the exponentials and error functions of each axis once
 // per end, and forms the directional fluxes of that edge from them.  Every
 // flux value is computed with the reference's own expressions, so it is
@@ -118,29 +119,70 @@ __device__ __forceinline__ double cramer_weight(const double* cf, int d, double 
     return wgt * (cf[3 * d] * dels + cf[3 * d + 1] * delt + cf[3 * d + 2] * deln);
 }
 
+// Half-range fluxes along axis A for the lists of that axis the entry is in:
+// ab bit 0 = list 2A (positive half), bit 1 = list 2A+1.  An interior
+// neighbour of conn is in exactly one of the two (generate_connectivity.f90:
+// 72-106 splits with <= 0 and > 0); a spilled entry may be in either.
+template <int A, typename T>
+__device__ __forceinline__ void half_axis(const EndState<T>& ei, const EndState<T>& ek, unsigned ab, T Bi, T eri,
+                                          T Bk, T erk, const double* cf, double wgt, double dels, double delt,
+                                          double deln, double acc[5]) {
+    T Gi[5], Gk[5];
+    const bool pos = ab & 1u;
+    const T sg = pos ? T(1) : T(-1);
+    half_flux<A>(ei, sg, Bi, eri, Gi);
+    half_flux<A>(ek, sg, Bk, erk, Gk);
+    accumulate(acc, Gi, Gk, cramer_weight(cf, 2 * A + (pos ? 0 : 1), wgt, dels, delt, deln));
+    if (ab == 3u) {
+        half_flux<A>(ei, T(-1), Bi, eri, Gi);
+        half_flux<A>(ek, T(-1), Bk, erk, Gk);
+        accumulate(acc, Gi, Gk, cramer_weight(cf, 2 * A + 1, wgt, dels, delt, deln));
+    }
+}
+
+// interior points on this kernel have every walk entry in exactly one list
+// per axis (checked at bind)
 template <int A, typename T>
 __device__ __forceinline__ void interior_axis(const EndState<T>& ei, const EndState<T>& ek, unsigned bits,
                                               const double* cf, double wgt, double dels, double delt,
                                               double deln, double acc[5]) {
-    // exactly one of the two lists of this axis holds the neighbour
-    // (generate_connectivity.f90:72-106: <= 0 and > 0)
     const bool pos = (bits >> (2 * A)) & 1u;
     const T sg = pos ? T(1) : T(-1);
-    const int d = 2 * A + (pos ? 0 : 1);
     T Bi, eri, Bk, erk, Gi[5], Gk[5];
     axis_terms<A>(ei, Bi, eri);
     axis_terms<A>(ek, Bk, erk);
     half_flux<A>(ei, sg, Bi, eri, Gi);
     half_flux<A>(ek, sg, Bk, erk, Gk);
-    accumulate(acc, Gi, Gk, cramer_weight(cf, d, wgt, dels, delt, deln));
+    accumulate(acc, Gi, Gk, cramer_weight(cf, 2 * A + (pos ? 0 : 1), wgt, dels, delt, deln));
 }
 
+// x and y half-range lists of an interior point on the general path
+template <typename T>
+__device__ __forceinline__ void half_axes_xy(const EndState<T>& ei, const EndState<T>& ek, unsigned bits,
+                                             const double* cf, double wgt, double dels, double delt, double deln,
+                                             double acc[5]) {
+    T Bi, eri, Bk, erk;
+    const unsigned abx = bits & 3u, aby = (bits >> 2) & 3u;
+    if (abx) {
+        axis_terms<0>(ei, Bi, eri);
+        axis_terms<0>(ek, Bk, erk);
+        half_axis<0>(ei, ek, abx, Bi, eri, Bk, erk, cf, wgt, dels, delt, deln, acc);
+    }
+    if (aby) {
+        axis_terms<1>(ei, Bi, eri);
+        axis_terms<1>(ek, Bk, erk);
+        half_axis<1>(ei, ek, aby, Bi, eri, Bk, erk, cf, wgt, dels, delt, deln, acc);
+    }
+}
+
+// Quarter-range fluxes along tangent axis A of a wall point (sn = -1, Gw)
+// or an outer point (sn = +1, Go).  Their x/y lists split with <= 0 and
+// >= 0 (generate_connectivity.f90:148-170, 216-238): a neighbour with a zero
+// offset sits in both.
 template <int A, typename T>
-__device__ __forceinline__ void wall_axis(const EndState<T>& ei, const EndState<T>& ek, unsigned bits, T B3i,
-                                          T er3i, T B3k, T er3k, const double* cf, double wgt, double dels,
-                                          double delt, double deln, double acc[5]) {
-    // wall lists split with <= 0 and >= 0 (generate_connectivity.f90:148-170):
-    // a neighbour with a zero offset sits in both
+__device__ __forceinline__ void octant_axis(const EndState<T>& ei, const EndState<T>& ek, unsigned bits, T sn,
+                                            T B3i, T er3i, T B3k, T er3k, const double* cf, double wgt,
+                                            double dels, double delt, double deln, double acc[5]) {
     const unsigned ab = (bits >> (2 * A)) & 3u;
     if (!ab) return;
     T Bi, eri, Bk, erk, Gi[5], Gk[5];
@@ -148,18 +190,23 @@ __device__ __forceinline__ void wall_axis(const EndState<T>& ei, const EndState<
     axis_terms<A>(ek, Bk, erk);
     const bool pos = ab & 1u;
     const T sg = pos ? T(1) : T(-1);
-    octant_flux<A>(ei, sg, T(-1), Bi, eri, B3i, er3i, Gi);
-    octant_flux<A>(ek, sg, T(-1), Bk, erk, B3k, er3k, Gk);
+    octant_flux<A>(ei, sg, sn, Bi, eri, B3i, er3i, Gi);
+    octant_flux<A>(ek, sg, sn, Bk, erk, B3k, er3k, Gk);
     accumulate(acc, Gi, Gk, cramer_weight(cf, 2 * A + (pos ? 0 : 1), wgt, dels, delt, deln));
     if (ab == 3u) {
-        octant_flux<A>(ei, T(-1), T(-1), Bi, eri, B3i, er3i, Gi);
-        octant_flux<A>(ek, T(-1), T(-1), Bk, erk, B3k, er3k, Gk);
+        octant_flux<A>(ei, T(-1), sn, Bi, eri, B3i, er3i, Gi);
+        octant_flux<A>(ek, T(-1), sn, Bk, erk, B3k, er3k, Gk);
         accumulate(acc, Gi, Gk, cramer_weight(cf, 2 * A + 1, wgt, dels, delt, deln));
     }
 }
 
-// CLASS 0: interior (6 half-range lists); CLASS 1: wall (Gw octant lists in
-// x and y, half-range Gzn over conn, factor 2 of flux_residual.f90:50).
+// CLASS 0: interior points whose walk is conn with every entry in exactly
+// one list per axis (6 half-range lists).  CLASS 1: every other point, told
+// apart by status: wall points take the Gw octant lists in x and y, the
+// half-range z- list over conn and the factor 2 of flux_residual.f90:50;
+// outer points the Go octant lists and the half-range z+ list (:54-66);
+// interior points whose lists the 22-column spill altered the six
+// half-range lists, each walk entry in the lists its bits name (:68-81).
 // T: arithmetic of the reconstruction and fluxes (double: the FP64 form;
 // float: the mixed form, reading the FP32 q, qm, dq); geometry, Cramer
 // weights and the sums stay FP64.
@@ -199,9 +246,11 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
     }
     const T epsi2_i = T(limiter_epsi2(fp.vl_const, pi4.w));
     const double* cf = g.coef + 18 * (long long)i;
-    const int nb = g.nbhs[i];
-    const int* cn = g.conn + kConnCols * (long long)i;
-    const unsigned char* mk = g.mask + kConnCols * (long long)i;
+    const int nb = g.nwalk[i];
+    const int* cn = g.walk + kWalkCols * (long long)i;
+    const unsigned char* mk = g.mask + kWalkCols * (long long)i;
+    const int st_i = CLASS == 1 ? g.status[i] : 0;
+    const T sn = st_i == 2 ? T(1) : T(-1);
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int j = lane; j < nb; j += kFluxGroup) {
         const unsigned bits = mk[j];
@@ -234,21 +283,21 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
             T B3i, er3i, B3k, er3k;
             axis_terms<2>(ei, B3i, er3i);
             axis_terms<2>(ek, B3k, er3k);
-            wall_axis<0>(ei, ek, bits, B3i, er3i, B3k, er3k, cf, wgt, dels, delt, deln, acc);
-            wall_axis<1>(ei, ek, bits, B3i, er3i, B3k, er3k, cf, wgt, dels, delt, deln, acc);
-            if (bits & (1u << ZNEG)) {
-                T Gi[5], Gk[5];
-                half_flux<2>(ei, T(-1), B3i, er3i, Gi);
-                half_flux<2>(ek, T(-1), B3k, er3k, Gk);
-                accumulate(acc, Gi, Gk, cramer_weight(cf, ZNEG, wgt, dels, delt, deln));
+            if (st_i == 0) {
+                half_axes_xy(ei, ek, bits, cf, wgt, dels, delt, deln, acc);
+            } else {
+                octant_axis<0>(ei, ek, bits, sn, B3i, er3i, B3k, er3k, cf, wgt, dels, delt, deln, acc);
+                octant_axis<1>(ei, ek, bits, sn, B3i, er3i, B3k, er3k, cf, wgt, dels, delt, deln, acc);
             }
+            const unsigned abz = (bits >> (2 * 2)) & 3u;
+            if (abz) half_axis<2>(ei, ek, abz, B3i, er3i, B3k, er3k, cf, wgt, dels, delt, deln, acc);
         }
     }
 #pragma unroll
     for (int r = 0; r < 5; ++r) acc[r] = group_sum<kFluxGroup>(acc[r]);
     if (lane == 0) {
 #pragma unroll
-        for (int r = 0; r < 5; ++r) s.flux_res[5 * (long long)i + r] = CLASS == 1 ? 2.0 * acc[r] : acc[r];
+        for (int r = 0; r < 5; ++r) s.flux_res[5 * (long long)i + r] = st_i == 1 ? 2.0 * acc[r] : acc[r];
     }
 }
 

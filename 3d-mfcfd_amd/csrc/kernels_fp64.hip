@@ -373,35 +373,49 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
     }
 }
 
-// flux_residual.f90:40-81 for the points of idx (any class, from status)
+// flux_residual.f90:40-81 for the points of idx (any class, from status).
+// A point is handled by kListGroup lanes: lane l evaluates the point's l-th
+// direction list (x+, x-, y+, y-, then z+ and z- for interior points, z-
+// over conn for wall points, z+ for outer points), and every lane of the
+// group then adds the list results in that order, as flux_residual.f90 does.
+// The lists of a point run side by side instead of one after another.
+constexpr int kListGroup = 8;
 template <bool UW, typename T>
 __global__ void __launch_bounds__(kBlock) k_flux_residual_lists(DevGeom g, DevState s, FlowParams fp,
                                                                 const int* idx, int count) {
-    const int m = xcd_index();
-    if (m >= count) return;
+    const unsigned t = (unsigned)xcd_index();
+    const int m = (int)(t / kListGroup);
+    const int lane = (int)(t % kListGroup);
+    if (m >= count) return;  // whole groups: blocks hold a multiple of kListGroup lanes
     const int i = idx[m];
     const int cls = g.status[i];
-    const Frame<double> f = load_frame(g.frame, i);
-    double acc[5], G[5];
-    stencil_dG<UW, T>(g, s, fp, i, dir_stencil(g, cls, XPOS, i), f, acc);
-    const int last = cls == 0 ? ZNEG : YNEG;
-#pragma unroll 1
-    for (int d = XNEG; d <= last; ++d) {
+    const int n_lists = cls == 0 ? 6 : (cls == 1 || cls == 2 ? 5 : 4);
+    double G[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (lane < n_lists) {
+        const int d = lane < 4 || cls == 0 ? lane : (cls == 1 ? ZNEG : ZPOS);
+        const Frame<double> f = load_frame(g.frame, i);
         stencil_dG<UW, T>(g, s, fp, i, dir_stencil(g, cls, d, i), f, G);
-#pragma unroll
-        for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
     }
-    if (cls == 1) {
-        stencil_dG<UW, T>(g, s, fp, i, dir_stencil(g, 1, ZNEG, i), f, G);
+    double acc[5], v4[5], v5[5];
 #pragma unroll
-        for (int r = 0; r < 5; ++r) acc[r] = 2.0 * (acc[r] + G[r]);
-    } else if (cls == 2) {
-        stencil_dG<UW, T>(g, s, fp, i, dir_stencil(g, 2, ZPOS, i), f, G);
+    for (int r = 0; r < 5; ++r) {
+        acc[r] = __shfl(G[r], 0, kListGroup);
 #pragma unroll
-        for (int r = 0; r < 5; ++r) acc[r] = acc[r] + G[r];
+        for (int l = 1; l < 4; ++l) acc[r] = acc[r] + __shfl(G[r], l, kListGroup);
+        v4[r] = __shfl(G[r], 4, kListGroup);
+        v5[r] = __shfl(G[r], 5, kListGroup);
     }
+    if (lane != 0) return;
 #pragma unroll
-    for (int r = 0; r < 5; ++r) s.flux_res[5 * (long long)i + r] = acc[r];
+    for (int r = 0; r < 5; ++r) {
+        if (cls == 0)
+            acc[r] = (acc[r] + v4[r]) + v5[r];
+        else if (cls == 1)
+            acc[r] = 2.0 * (acc[r] + v4[r]);
+        else if (cls == 2)
+            acc[r] = acc[r] + v4[r];
+        s.flux_res[5 * (long long)i + r] = acc[r];
+    }
 }
 
 // explicit update through the sweep's outputs: delUn = -delt*R, delUp = delUn
@@ -921,12 +935,11 @@ template <typename T>
 void launch_lists(const DevGeom& g, const DevState& s, const FlowParams& p, const int* idx, int count,
                   hipStream_t st) {
     if (count <= 0) return;
+    const dim3 grid(grid_for((long long)count * kListGroup, kBlock));
     if (p.power == 0.0)
-        hipLaunchKernelGGL((k_flux_residual_lists<true, T>), dim3(grid_for(count, kBlock)), dim3(kBlock), 0, st, g,
-                           s, p, idx, count);
+        hipLaunchKernelGGL((k_flux_residual_lists<true, T>), grid, dim3(kBlock), 0, st, g, s, p, idx, count);
     else
-        hipLaunchKernelGGL((k_flux_residual_lists<false, T>), dim3(grid_for(count, kBlock)), dim3(kBlock), 0, st, g,
-                           s, p, idx, count);
+        hipLaunchKernelGGL((k_flux_residual_lists<false, T>), grid, dim3(kBlock), 0, st, g, s, p, idx, count);
 }
 void launch_flux_residual(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                           hipStream_t st, int parts, const LaunchShapes& ls) {

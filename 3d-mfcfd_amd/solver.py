@@ -303,10 +303,11 @@ class Solver:
     def point_classes(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _check(lib().mfcfd_point_classes(self._ctx, C.byref(a), C.byref(b), C.byref(c)))
-        return {"fused_interior": a.value, "fused_wall": b.value, "listwise": c.value}
+        return {"fused_interior": a.value, "fused_general": b.value, "listwise": c.value}
 
     def point_class_map(self):
-        """Per host point: 0 fused interior, 1 fused wall, 2 list-by-list."""
+        """Per host point: 0 fused interior, 1 fused general (wall, outer, spill-altered
+        interior), 2 list-by-list."""
         out = np.zeros(self.n, dtype=np.int32)
         _check(lib().mfcfd_point_class_map(self._ctx, out.ctypes.data_as(_ip)))
         return out

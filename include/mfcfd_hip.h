@@ -241,13 +241,15 @@ int mfcfd_time_phases(mfcfd_ctx *ctx, int t0, int n_iters, double *out_ms);
  * Gauss-Seidel dependency chains) and swept point count */
 int mfcfd_sweep_info(mfcfd_ctx *ctx, int *levels_fwd, int *levels_bwd, int *n_swept);
 
-/* diagnostics: how many points take the fused flux kernel (interior, wall)
- * and how many the list-by-list kernel (outer points and points whose lists
- * were altered by the 22-column spill) */
+/* diagnostics: how many points take the fused flux kernel's interior class
+ * (interior points whose lists partition conn), its general class (wall and
+ * outer points, and interior points whose lists the 22-column spill
+ * altered) and the list-by-list kernel (points whose conn and out-of-order
+ * list entries together exceed 32) */
 int mfcfd_point_classes(mfcfd_ctx *ctx, int *n_fused_interior, int *n_fused_wall, int *n_listwise);
 
 /* diagnostics: the same classes per point, host order (N ints):
- * 0 fused interior, 1 fused wall, 2 list-by-list */
+ * 0 fused interior, 1 fused general, 2 list-by-list */
 int mfcfd_point_class_map(mfcfd_ctx *ctx, int *cls);
 
 /* diagnostics: with timing enabled, every iteration records HIP events on

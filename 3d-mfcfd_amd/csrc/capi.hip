@@ -128,7 +128,7 @@ struct mfcfd_ctx {
     DevBuf<int> idx_fused_interior, idx_fused_wall, idx_generic;
     TypeLists tl{};
     // state
-    DevBuf<double> prim, U, q, qm, dq, delt, flux_res, delUp, delUn, dinv, temp, staging;
+    DevBuf<double> prim, U, q, qm, dq, delt, tsden, flux_res, delUp, delUn, dinv, temp, staging;
     DevBuf<float> q32, qm32, dq32;   // mixed precision
     // residual
     DevBuf<double> partial;
@@ -202,6 +202,7 @@ struct mfcfd_ctx {
         s.qm = qm.p;
         s.dq = dq.p;
         s.delt = delt.p;
+        s.tsden = tsden.p;
         s.flux_res = flux_res.p;
         s.delUp = delUp.p;
         s.delUn = delUn.p;
@@ -212,7 +213,7 @@ struct mfcfd_ctx {
         return s;
     }
     void release_all() {
-        DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &q, &qm, &dq, &delt, &flux_res, &delUp, &delUn,
+        DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &q, &qm, &dq, &delt, &tsden, &flux_res, &delUp, &delUn,
                                  &dinv, &temp, &staging, &partial, &hist};
         for (auto* b : dbl) b->release();
         DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &walk, &nwalk,
@@ -772,7 +773,8 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
         HIP_TRY(hipMemsetAsync(b->p, 0, 5ULL * n * sizeof(double), st));
     }
     if (c->qm.alloc(10ULL * n) != hipSuccess || c->dq.alloc(15ULL * n) != hipSuccess ||
-        c->delt.alloc((size_t)n) != hipSuccess || c->dinv.alloc((size_t)n) != hipSuccess ||
+        c->delt.alloc((size_t)n) != hipSuccess || c->tsden.alloc((size_t)n) != hipSuccess ||
+        c->dinv.alloc((size_t)n) != hipSuccess ||
         c->staging.alloc(15ULL * n) != hipSuccess)
         return fail(MFCFD_ENOMEM, "state arrays");
     HIP_TRY(hipMemsetAsync(c->qm.p, 0, 10ULL * n * sizeof(double), st));

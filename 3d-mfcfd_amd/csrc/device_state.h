@@ -70,6 +70,7 @@ struct DevState {
     double* qm;
     double* dq;
     double* delt;
+    double* tsden;   // [n] |u| + 3 sqrt(p/rho) of each point, the local time step's denominator
     double* flux_res;
     double* delUp;
     double* delUn;
@@ -80,23 +81,28 @@ struct DevState {
     float* dq32;
 };
 
-// q-variable arrays of one precision
+// q-variable arrays of one precision: point k's q (5), qm (2,5) and dq (3,5)
+// start at q + sq*k, qm + sqm*k, dq + sdq*k
 template <typename T>
 struct QArrays {
     T* q;
     T* qm;
     T* dq;
+    int sq, sqm, sdq;
+    __host__ __device__ T* qp(long long k) const { return q + sq * k; }
+    __host__ __device__ T* qmp(long long k) const { return qm + sqm * k; }
+    __host__ __device__ T* dqp(long long k) const { return dq + sdq * k; }
 };
 
 template <typename T>
 __host__ __device__ inline QArrays<T> qarrays(const DevState& s);
 template <>
 __host__ __device__ inline QArrays<double> qarrays<double>(const DevState& s) {
-    return {s.q, s.qm, s.dq};
+    return {s.q, s.qm, s.dq, 5, 10, 15};
 }
 template <>
 __host__ __device__ inline QArrays<float> qarrays<float>(const DevState& s) {
-    return {s.q32, s.qm32, s.dq32};
+    return {s.q32, s.qm32, s.dq32, 5, 10, 15};
 }
 
 struct FlowParams {

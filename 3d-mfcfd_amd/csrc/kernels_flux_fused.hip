@@ -53,9 +53,9 @@ __device__ __forceinline__ EndState<T> end_state(const T prim[5], const Frame<T>
     e.ut[0] = f.t1[0] * u1 + f.t1[1] * u2 + f.t1[2] * u3;
     e.ut[1] = f.t2[0] * u1 + f.t2[1] * u2 + f.t2[2] * u3;
     e.ut[2] = f.n[0] * u1 + f.n[1] * u2 + f.n[2] * u3;
-    const T beta = T(0.5) * e.rho / e.pr;
-    e.sqrt_beta = dsqrt(beta);
-    e.sqrt_pi_beta = dsqrt(pi * beta);
+    const T beta = ddiv(T(0.5) * e.rho, e.pr);
+    e.sqrt_beta = dsqrt_fast(beta);
+    e.sqrt_pi_beta = dsqrt_fast(pi * beta);
     const T u_sqr = e.ut[0] * e.ut[0] + e.ut[1] * e.ut[1] + e.ut[2] * e.ut[2];
     e.e5 = T(2.5) * e.pr + T(0.5) * e.rho * u_sqr;
     return e;
@@ -65,8 +65,9 @@ __device__ __forceinline__ EndState<T> end_state(const T prim[5], const Frame<T>
 template <int A, typename T>
 __device__ __forceinline__ void axis_terms(const EndState<T>& e, T& B, T& er) {
     const T S = e.ut[A] * e.sqrt_beta;
-    B = T(0.5) * dexp(-S * S) / e.sqrt_pi_beta;
-    er = derf(S);
+    const T ex = dexp(-S * S);
+    B = ddiv(T(0.5) * ex, e.sqrt_pi_beta);
+    er = derf_e(S, ex);
 }
 
 // split_fluxes.f90 flux_G{x,y,z}{p,n}; sg = +1 or -1
@@ -226,13 +227,14 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
     const T pi = T(fp.pi);
     const double4 pi4 = g.pos[i];
     // the point's own reconstruction inputs: FP32 keeps them in registers
-    // (re-reading them for every neighbour doubles the requests to L2); the
-    // FP64 kernel has no registers to spare and re-reads them
+    // (re-reading them for every neighbour doubles the gather's load
+    // requests); in FP64 they would double the kernel's registers and halve
+    // its resident waves, which costs more (measured), so it re-reads them
     constexpr bool kOwnInRegs = sizeof(T) == 4;
     T own[kOwnInRegs ? 30 : 1];
-    const T* qip = qa.q + 5 * (long long)i;
-    const T* dqi = qa.dq + 15 * (long long)i;
-    const T* qmi = qa.qm + 10 * (long long)i;
+    const T* qip = qa.qp(i);
+    const T* dqi = qa.dqp(i);
+    const T* qmi = qa.qmp(i);
     if constexpr (kOwnInRegs) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) own[r] = qip[r];
@@ -271,9 +273,9 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
         const EndState<T> ei = end_state(prim, ft, pi);
         T qk[5];
 #pragma unroll
-        for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
-        reconstruct_end<T>(qk, qa.dq + 15 * (long long)k, qa.qm + 10 * (long long)k,
-                           T(limiter_epsi2(fp.vl_const, pk.w)), T(delx), T(dely), T(delz), prim);
+        for (int r = 0; r < 5; ++r) qk[r] = qa.qp(k)[r];
+        reconstruct_end<T>(qk, qa.dqp(k), qa.qmp(k), T(limiter_epsi2(fp.vl_const, pk.w)), T(delx), T(dely), T(delz),
+                           prim);
         const EndState<T> ek = end_state(prim, ft, pi);
         if (CLASS == 0) {
             interior_axis<0>(ei, ek, bits, cf, wgt, dels, delt, deln, acc);

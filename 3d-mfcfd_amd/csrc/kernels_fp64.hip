@@ -115,14 +115,25 @@ __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
         q[2] = two_times_beta * u2;
         q[3] = two_times_beta * u3;
         q[4] = -two_times_beta;
-        QT* qo = qarrays<QT>(s).q + 5 * (long long)k;
+        QT* qo = qarrays<QT>(s).qp(k);
 #pragma unroll
         for (int r = 0; r < 5; ++r) qo[r] = QT(q[r]);
+        // timestep_delt.f90:40-42: the neighbour-only part of the local step
+        s.tsden[k] = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3) + 3.0 * __builtin_sqrt(pr / rho);
     }
 }
 
+// the time step's denominators alone (time step called outside an iteration)
+__global__ void __launch_bounds__(kBlock) k_timestep_den(DevGeom g, DevState s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.n) return;
+    const double* p = s.prim + 5 * (long long)k;
+    const double rho = p[0], u1 = p[1], u2 = p[2], u3 = p[3], pr = p[4];
+    s.tsden[k] = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3) + 3.0 * __builtin_sqrt(pr / rho);
+}
+
 // ---------------------------------------------------------------------------
-// q_derivatives.f90:17-110 (first pass) and timestep_delt.f90:13-38
+// q_derivatives.f90:17-110 (first pass) and timestep_delt.f90:20-52
 // ---------------------------------------------------------------------------
 // QT: type of q, qm, dq and of the gradient right-hand sides (double: the
 // FP64 form; float: the mixed form); the geometric sums and the 3x3 solve
@@ -149,7 +160,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
     if (DO_DQ) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            qi[r] = qa.q[5 * (long long)i + r];
+            qi[r] = qa.qp(i)[r];
             qmax[r] = qi[r];
             qmin[r] = qi[r];
             sx[r] = sy[r] = sz[r] = QT(0);
@@ -165,7 +176,7 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
         if (DO_DQ) {
             QT qk[5];
 #pragma unroll
-            for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
+            for (int r = 0; r < 5; ++r) qk[r] = qa.qp(k)[r];
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 if (qk[r] > qmax[r]) qmax[r] = qk[r];
@@ -188,11 +199,9 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
             }
         }
         if (DO_DT) {
+            // timestep_delt.f90:34-47, the neighbour's denominator read, not recomputed
             const double dist = __builtin_sqrt(delx * delx + dely * dely + delz * delz);
-            const double* pk5 = s.prim + 5 * (long long)k;
-            const double rho = pk5[0], u1 = pk5[1], u2 = pk5[2], u3 = pk5[3], pr = pk5[4];
-            const double mod_u = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3);
-            const double delt = fp.cfl * dist / (mod_u + 3.0 * __builtin_sqrt(pr / rho));
+            const double delt = fp.cfl * dist / s.tsden[k];
             if (delt < min_delt) min_delt = delt;
         }
     }
@@ -219,11 +228,11 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
         const double one_by_det = 1.0 / det;
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            qa.qm[(5 * (long long)i + r) * 2 + 0] = qmax[r];
-            qa.qm[(5 * (long long)i + r) * 2 + 1] = qmin[r];
+            qa.qmp(i)[r * 2 + 0] = qmax[r];
+            qa.qmp(i)[r * 2 + 1] = qmin[r];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                qa.dq[(5 * (long long)i + r) * 3 + c] =
+                qa.dqp(i)[r * 3 + c] =
                     QT(cramer_num(geo, c, (double)sx[r], (double)sy[r], (double)sz[r]) * one_by_det);
         }
     }
@@ -326,7 +335,7 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
     const double4 pi4 = g.pos[i];
     T qi[5];
 #pragma unroll
-    for (int r = 0; r < 5; ++r) qi[r] = qa.q[5 * (long long)i + r];
+    for (int r = 0; r < 5; ++r) qi[r] = qa.qp(i)[r];
     const T epsi2_i = T(limiter_epsi2(fp.vl_const, pi4.w));
     for (int j = 0; j < st.count; ++j) {
         const int k = st.list[j];
@@ -346,12 +355,12 @@ __device__ void stencil_dG(const DevGeom& g, const DevState& s, const FlowParams
         geo.yz = geo.yz + delt * deln_w;
         geo.zx = geo.zx + deln * dels_w;
         T prim[5], Gi[5], Gk[5], qk[5];
-        reconstruct_end<T>(qi, qa.dq + 15 * (long long)i, qa.qm + 10 * (long long)i, epsi2_i, T(delx), T(dely),
+        reconstruct_end<T>(qi, qa.dqp(i), qa.qmp(i), epsi2_i, T(delx), T(dely),
                            T(delz), prim);
         flux_eval<T>(st.family, st.axis, st.sign, ft, prim, T(fp.pi), Gi);
 #pragma unroll
-        for (int r = 0; r < 5; ++r) qk[r] = qa.q[5 * (long long)k + r];
-        reconstruct_end<T>(qk, qa.dq + 15 * (long long)k, qa.qm + 10 * (long long)k,
+        for (int r = 0; r < 5; ++r) qk[r] = qa.qp(k)[r];
+        reconstruct_end<T>(qk, qa.dqp(k), qa.qmp(k),
                            T(limiter_epsi2(fp.vl_const, pk.w)), T(delx), T(dely), T(delz), prim);
         flux_eval<T>(st.family, st.axis, st.sign, ft, prim, T(fp.pi), Gk);
 #pragma unroll
@@ -904,6 +913,7 @@ void launch_q_derivatives(const DevGeom& g, const DevState& s, const FlowParams&
 }
 void launch_timestep(const DevGeom& g, const DevState& s, const FlowParams& p, hipStream_t st) {
     if (g.n <= 0) return;
+    hipLaunchKernelGGL(k_timestep_den, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, s);
     // no least-squares weights in the time step
     hipLaunchKernelGGL((k_qderiv_timestep<false, true, true, double>), dim3(grid_for((long long)g.n * kQdGroup, kBlock)),
                        dim3(kBlock), 0, st, g, s, p);

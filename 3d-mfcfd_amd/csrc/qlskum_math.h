@@ -5,7 +5,8 @@
 // -ffp-contract=off, so the only differences from the reference CPU build
 // come from the device exp/log/erf (ocml vs glibc, within an ulp or two).
 // Routines are templated on the arithmetic type T so the mixed-precision
-// kernels reuse them with float.
+// kernels reuse them with float, where division, square root and erf take
+// the faster forms defined below.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,19 +16,52 @@ namespace mfcfd {
 // limiters.f90:26,28 compare |del_neg| with the REAL(4) literal 10e-6
 constexpr double kLimiterThreshold = (double)10e-6f;
 
-template <typename T> __device__ __forceinline__ T dsqrt(T x);
-template <> __device__ __forceinline__ double dsqrt<double>(double x) { return __builtin_sqrt(x); }
-template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
 template <typename T> __device__ __forceinline__ T dexp(T x);
 template <> __device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ float dexp<float>(float x) { return __expf(x); }
 template <typename T> __device__ __forceinline__ T dlog(T x);
 template <> __device__ __forceinline__ double dlog<double>(double x) { return log(x); }
 template <> __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
-template <typename T> __device__ __forceinline__ T derf(T x);
-template <> __device__ __forceinline__ double derf<double>(double x) { return erf(x); }
-template <> __device__ __forceinline__ float derf<float>(float x) { return erff(x); }
 template <typename T> __device__ __forceinline__ T dabs(T x) { return x < T(0) ? -x : x; }
+
+// Division, square root and error function of the arithmetic type T.  FP64
+// keeps IEEE division and square root and the library erf: the reference's
+// arithmetic.  The mixed form's FP32 takes the hardware reciprocal and
+// square root (v_rcp_f32, v_sqrt_f32: 1 ulp) and its own erf below; every
+// argument here is a normal number.
+template <typename T> __device__ __forceinline__ T ddiv(T a, T b);
+template <> __device__ __forceinline__ double ddiv<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float ddiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename T> __device__ __forceinline__ T dsqrt_fast(T x);
+template <> __device__ __forceinline__ double dsqrt_fast<double>(double x) { return __builtin_sqrt(x); }
+template <> __device__ __forceinline__ float dsqrt_fast<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// erf(x), given ex = exp(-x*x), which every flux caller forms anyway.
+// FP32: for |x| < 0.5 the Maclaurin series of erf to x^13 (truncation
+// < 2e-8); above it Abramowitz & Stegun 7.1.26, erf = 1 - t*(a1 + t*(a2 +
+// t*(a3 + t*(a4 + t*a5))))*ex with t = 1/(1 + p|x|) (|error| <= 1.5e-7).
+// In FP32 arithmetic the pair stays within 4e-7 of erf over all x; both
+// branches are evaluated and one selected, so lanes do not diverge.
+template <typename T> __device__ __forceinline__ T derf_e(T x, T ex);
+template <> __device__ __forceinline__ double derf_e<double>(double x, double) { return erf(x); }
+template <> __device__ __forceinline__ float derf_e<float>(float x, float ex) {
+    const float ax = x < 0.0f ? -x : x;
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+    const float p =
+        t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    const float far = 1.0f - p * ex;
+    // 2/sqrt(pi) * sum_n (-1)^n x^(2n+1) / (n! (2n+1)), n = 0..6
+    const float x2 = ax * ax;
+    float s = 1.2055332981789664e-4f;
+    s = s * x2 - 8.548327023450852e-4f;
+    s = s * x2 + 5.223977625442188e-3f;
+    s = s * x2 - 2.6866170645131252e-2f;
+    s = s * x2 + 1.1283791670955126e-1f;
+    s = s * x2 - 3.7612638903183754e-1f;
+    s = s * x2 + 1.1283791670955126f;
+    const float r = ax < 0.5f ? ax * s : far;
+    return x < 0.0f ? -r : r;
+}
 
 // The six symmetric sums of a weighted least-squares stencil and the 3x3
 // Cramer solve every stencil of the reference uses
@@ -75,11 +109,12 @@ __device__ __forceinline__ void flux_half(int axis, int sign, const Frame<T>& f,
     const T ut1 = f.t1[0] * u1 + f.t1[1] * u2 + f.t1[2] * u3;
     const T ut2 = f.t2[0] * u1 + f.t2[1] * u2 + f.t2[2] * u3;
     const T un = f.n[0] * u1 + f.n[1] * u2 + f.n[2] * u3;
-    const T beta = T(0.5) * rho / pr;
+    const T beta = ddiv(T(0.5) * rho, pr);
     const T ua = axis == 0 ? ut1 : (axis == 1 ? ut2 : un);
-    const T S = ua * dsqrt(beta);
-    const T B = T(0.5) * dexp(-S * S) / dsqrt(pi * beta);
-    const T e = derf(S);
+    const T S = ua * dsqrt_fast(beta);
+    const T ex = dexp(-S * S);
+    const T B = ddiv(T(0.5) * ex, dsqrt_fast(pi * beta));
+    const T e = derf_e(S, ex);
     const T A = T(0.5) * (sign > 0 ? T(1) + e : T(1) - e);
     const T Bs = sign > 0 ? B : -B;
     const T u_sqr = ut1 * ut1 + ut2 * ut2 + un * un;
@@ -103,15 +138,16 @@ __device__ __forceinline__ void flux_octant(int axis, int sign_a, int sign_n, co
     const T ut1 = f.t1[0] * u1 + f.t1[1] * u2 + f.t1[2] * u3;
     const T ut2 = f.t2[0] * u1 + f.t2[1] * u2 + f.t2[2] * u3;
     const T un = f.n[0] * u1 + f.n[1] * u2 + f.n[2] * u3;
-    const T beta = T(0.5) * rho / pr;
+    const T beta = ddiv(T(0.5) * rho, pr);
     const T ua = axis == 0 ? ut1 : ut2;
-    const T Sa = ua * dsqrt(beta);
-    const T S3 = un * dsqrt(beta);
-    const T Ba = T(0.5) * dexp(-Sa * Sa) / dsqrt(pi * beta);
-    const T B3 = T(0.5) * dexp(-S3 * S3) / dsqrt(pi * beta);
-    const T ea = derf(Sa);
+    const T Sa = ua * dsqrt_fast(beta);
+    const T S3 = un * dsqrt_fast(beta);
+    const T exa = dexp(-Sa * Sa), ex3 = dexp(-S3 * S3);
+    const T Ba = ddiv(T(0.5) * exa, dsqrt_fast(pi * beta));
+    const T B3 = ddiv(T(0.5) * ex3, dsqrt_fast(pi * beta));
+    const T ea = derf_e(Sa, exa);
     const T Aa = T(0.5) * (sign_a > 0 ? T(1) + ea : T(1) - ea);
-    const T e3 = derf(S3);
+    const T e3 = derf_e(S3, ex3);
     const T A3 = T(0.5) * (sign_n > 0 ? T(1) + e3 : T(1) - e3);
     const T u_sqr = ut1 * ut1 + ut2 * ut2 + un * un;
     const T temp1 = (ua * Aa + (sign_a > 0 ? Ba : -Ba));
@@ -153,7 +189,7 @@ __device__ __forceinline__ T venkat_phi_dn(T del_neg, T q, T qmax, T qmin, T eps
     T den = del_pos * del_pos + T(2) * del_neg * del_neg;
     den = den + del_neg * del_pos + epsi2;
     den = den * del_neg;
-    const T temp = num / den;
+    const T temp = ddiv(num, den);
     return temp < T(1) ? temp : T(1);
 }
 
@@ -166,7 +202,7 @@ __device__ __forceinline__ T venkat_phi(T qt, T q, T qmax, T qmin, T epsi2) {
 template <typename T>
 __device__ __forceinline__ void qtilde_to_primitive(const T q[5], T prim[5]) {
     const T beta = -q[4] * T(0.5);
-    const T temp = T(0.5) / beta;
+    const T temp = ddiv(T(0.5), beta);
     prim[1] = q[1] * temp;
     prim[2] = q[2] * temp;
     prim[3] = q[3] * temp;

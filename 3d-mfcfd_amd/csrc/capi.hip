@@ -377,10 +377,20 @@ int do_sweeps(mfcfd_ctx* c) {
     return launch_error("implicit sweeps");
 }
 
-int do_state_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
+int do_state_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr,
+                    bool with_explicit_sweep = false) {
     launch_state_update(c->geom(), c->state(), c->fp, c->tl, c->partial.p, c->partial_pos.p, c->n_partial,
-                        c->rec.p, hist, hist_index, c->stream, ir);
+                        c->rec.p, hist, hist_index, c->stream, ir, with_explicit_sweep);
     return launch_error("state update");
+}
+
+// the sweeps and the state update of one iteration; the explicit form's
+// sweep runs inside the state update's pass
+int do_sweeps_and_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
+    if (c->params.mode == MFCFD_EXPLICIT) return do_state_update(c, hist, hist_index, ir, true);
+    int rc = do_sweeps(c);
+    if (rc) return rc;
+    return do_state_update(c, hist, hist_index, ir);
 }
 
 int do_iteration(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
@@ -396,8 +406,7 @@ int do_iteration(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResid
     flux(c, kFluxAll);
     int rc = launch_error("iteration");
     if (rc) return rc;
-    if ((rc = do_sweeps(c))) return rc;
-    return do_state_update(c, hist, hist_index, ir);
+    return do_sweeps_and_update(c, hist, hist_index, ir);
 }
 
 // fpi_solver.f90:38-43
@@ -1475,9 +1484,11 @@ int mfcfd_time_phases(mfcfd_ctx* c, int t0, int n_iters, double* out_ms) {
         }
         hipEventRecord(ev[4], c->stream);
         if ((rc = launch_error("phases"))) break;
-        if ((rc = do_sweeps(c))) break;
+        // explicit form: the sweep is part of the state update's pass
+        const bool fused = c->params.mode == MFCFD_EXPLICIT;
+        if (!fused && (rc = do_sweeps(c))) break;
         hipEventRecord(ev[5], c->stream);
-        if ((rc = do_state_update(c, nullptr, 0))) break;
+        if ((rc = do_state_update(c, nullptr, 0, nullptr, fused))) break;
         hipEventRecord(ev[6], c->stream);
         if (hipEventSynchronize(ev[6]) != hipSuccess) {
             rc = fail(MFCFD_EHIP, "event synchronize");

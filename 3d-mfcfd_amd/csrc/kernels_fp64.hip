@@ -660,6 +660,10 @@ __device__ __forceinline__ MaxPair max_pair(MaxPair a, MaxPair b) {
     return a;
 }
 
+// EXPLICIT: the explicit form's sweep (k_explicit_sweep: delUn = delUp =
+// -delt*R for every point) is done here first, with the same arithmetic, so
+// delUn and delUp are written once instead of written and read back
+template <bool EXPLICIT>
 __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, FlowParams fp,
                                                          double* partial, int* partial_pos,
                                                          int* partial_pt, double* partial_max,
@@ -673,12 +677,26 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
     MaxPair mp = {0.0, 0x7fffffff, 0};
     if (k < g.n) {
         const int st = g.status[k];
+        double dU[5];
+        if (EXPLICIT) {
+            const double dt = s.delt[k];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                dU[r] = -(dt * s.flux_res[5 * (long long)k + r]);
+                s.delUn[5 * (long long)k + r] = dU[r];
+                s.delUp[5 * (long long)k + r] = dU[r];
+            }
+        }
         if (st == 0 || st == 1) {
             const Frame<double> f = load_frame(g.frame, k);
             const double temp = s.U[5 * (long long)k];
+            if (!EXPLICIT) {
+#pragma unroll
+                for (int r = 0; r < 5; ++r) dU[r] = s.delUn[5 * (long long)k + r];
+            }
             double U[5];
 #pragma unroll
-            for (int r = 0; r < 5; ++r) U[r] = s.U[5 * (long long)k + r] + s.delUn[5 * (long long)k + r];
+            for (int r = 0; r < 5; ++r) U[r] = s.U[5 * (long long)k + r] + dU[r];
             if (st == 1) U[3] = 0.0;
             res_sqr = (U[0] - temp) * (U[0] - temp);
             mp.v = res_sqr;
@@ -689,7 +707,7 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 s.prim[5 * (long long)k + r] = prim[r];
-                s.delUp[5 * (long long)k + r] = s.delUn[5 * (long long)k + r];
+                if (!EXPLICIT) s.delUp[5 * (long long)k + r] = dU[r];
             }
         } else if (st == 2) {
             const Frame<double> f = load_frame(g.frame, k);
@@ -1012,12 +1030,16 @@ void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams&
 int state_update_blocks(int n) { return grid_for(n, kBlock); }
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                          double* partial, int* partial_pos, int n_partial, ResidualRecord* rec, double* hist,
-                         int hist_index, hipStream_t st, const InstanceResidual* ir) {
+                         int hist_index, hipStream_t st, const InstanceResidual* ir, bool with_explicit_sweep) {
     // partial buffer layout: [sum | max] doubles, [pos | point] ints
     double* partial_max = partial + n_partial;
     int* partial_pt = partial_pos + n_partial;
-    hipLaunchKernelGGL(k_state_update, dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial, partial_pos,
-                       partial_pt, partial_max, ir ? ir->ressq : nullptr);
+    if (with_explicit_sweep)
+        hipLaunchKernelGGL(k_state_update<true>, dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial, partial_pos,
+                           partial_pt, partial_max, ir ? ir->ressq : nullptr);
+    else
+        hipLaunchKernelGGL(k_state_update<false>, dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
+                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
     if (tl.n_sup_in + tl.n_sup_out > 0)
         hipLaunchKernelGGL(k_supersonic, dim3(1), dim3(1), 0, st, g, s, tl.sup_out, tl.n_sup_out, tl.sup_in,
                            tl.n_sup_in);

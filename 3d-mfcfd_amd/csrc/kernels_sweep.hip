@@ -21,7 +21,7 @@
 // computes everything that does not depend on this sweep's values
 // (geometry, Cramer weights, the old fluxes Gold), then polls its
 // dependencies' values with agent-scope relaxed atomic loads until none is
-// pending, and reads them the same way.  Each 64-bit value is written once
+// pending, and uses the values so read.  Each 64-bit value is written once
 // and checked on its own, so no ordering between locations -- and no cache
 // write-back or invalidation -- is needed.  A wait that exceeds the spin
 // limit is counted in the residual record and the host reports an error;
@@ -56,33 +56,29 @@ constexpr int kSpinLimit = 1 << 20;
 // a signalling NaN with a payload: arithmetic only ever produces quiet NaNs
 constexpr unsigned long long kPending = 0x7FF4DEAD5EEDF00DULL;
 
-__device__ __forceinline__ double load_value(const double* p) {
-    const unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)b);
-}
-
 __device__ __forceinline__ void store_value(double* p, double v) {
     __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ bool all_written(const double* v5) {
-    bool ok = true;
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-        ok = ok && __hip_atomic_load(reinterpret_cast<const unsigned long long*>(v5 + r), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) != kPending;
-    return ok;
-}
-
-// wait until all five values of a dependency are written in this sweep;
-// gives up after the spin limit, or at once when another wait of this sweep
-// already gave up, so a broken schedule ends the kernel instead of stalling
-// it point by point
-__device__ __forceinline__ void wait_done(const double* v5, int* stalls) {
+// read the five values of a neighbour, waiting until none is pending: a
+// neighbour swept earlier in this sweep has then written them, any other
+// holds a value from before the sweep, which is never pending.  Gives up
+// after the spin limit, or at once when another wait of this sweep already
+// gave up, so a broken schedule ends the kernel instead of stalling it point
+// by point.
+__device__ __forceinline__ void wait_read(const double* v5, double out[5], int* stalls) {
     int spins = 0;
-    while (!all_written(v5)) {
+    for (;;) {
+        bool ok = true;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(v5 + r),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            out[r] = __longlong_as_double((long long)b);
+            ok = ok && b != kPending;
+        }
+        if (ok) return;
         ++spins;
         if ((spins & 255) == 0 && __hip_atomic_load(stalls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
             return;
@@ -131,7 +127,6 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         if (cls == 1) cnt[ZPOS] = 0;  // a wall point's z term walks conn (the ZNEG list)
 #pragma unroll
         for (int d = 0; d < 6; ++d) total += cnt[d];
-        const long long db = a.dep_ptr[t], de = a.dep_ptr[t + 1];
         // the point's own inputs of the final update, read before the wait
         double D_inv = 0.0, own[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (tid == 0) {
@@ -174,17 +169,15 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                     fk = load_frame(g.frame, k);
                 }
             }
-            if (base == 0) {
-                // wait for every dependency of this sweep position
-                for (long long q = db + tid; q < de; q += kSweepBlock)
-                    wait_done(dU + 5 * (long long)a.dep[q], a.stalls);
-                __syncthreads();
-            }
             if (take) {
-                double U[5], prim[5];
+                // the lanes that take an entry are exactly this point's
+                // dependencies in the schedule: each waits for its own
+                // neighbour and uses the values it polled
+                double U[5], prim[5], dk[5];
                 T pt[5], Gnew[5];
+                wait_read(dU + 5 * (long long)k, dk, a.stalls);
 #pragma unroll
-                for (int r = 0; r < 5; ++r) U[r] = load_value(dU + 5 * (long long)k + r) + Uk[r];
+                for (int r = 0; r < 5; ++r) U[r] = dk[r] + Uk[r];
                 conserved_to_prim(U, fk, prim);
 #pragma unroll
                 for (int r = 0; r < 5; ++r) pt[r] = T(prim[r]);
@@ -192,10 +185,6 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
 #pragma unroll
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }
-        }
-        if (total == 0) {
-            // no entries: still honour the dependency wait order
-            for (long long q = db + tid; q < de; q += kSweepBlock) wait_done(dU + 5 * (long long)a.dep[q], a.stalls);
         }
 #pragma unroll
         for (int r = 0; r < 5; ++r) acc[r] = wave_sum(acc[r]);

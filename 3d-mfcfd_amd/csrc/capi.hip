@@ -142,8 +142,7 @@ struct mfcfd_ctx {
     std::vector<int> fwd_ptr, bwd_ptr;
     DevBuf<int> fwd_points, bwd_points;
     // sync-free sweeps: points in sweep order, dependency lists, counters
-    DevBuf<int> fwd_seq, bwd_seq, fwd_dep, bwd_dep;
-    DevBuf<long long> fwd_dep_ptr, bwd_dep_ptr;
+    DevBuf<int> fwd_seq, bwd_seq;
     DevBuf<unsigned> sweep_ctr;
     // clouds of a batch (mfcfd_bind_batch): instance of each device point,
     // freestream (u1_inf, u2_inf) per instance, device-order ranges; a single
@@ -220,10 +219,8 @@ struct mfcfd_ctx {
                                &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
                                &idx_fused_wall, &idx_generic, &partial_pos, &fwd_points, &bwd_points,
-                               &fwd_seq, &bwd_seq, &fwd_dep, &bwd_dep};
+                               &fwd_seq, &bwd_seq};
         for (auto* b : ints) b->release();
-        fwd_dep_ptr.release();
-        bwd_dep_ptr.release();
         q32.release();
         qm32.release();
         dq32.release();
@@ -367,11 +364,7 @@ int do_sweeps(mfcfd_ctx* c) {
     fwd.ctr = bwd.ctr = c->sweep_ctr.p;
     fwd.stalls = bwd.stalls = &c->rec.p->stalls;
     fwd.seq = c->fwd_seq.p;
-    fwd.dep_ptr = c->fwd_dep_ptr.p;
-    fwd.dep = c->fwd_dep.p;
     bwd.seq = c->bwd_seq.p;
-    bwd.dep_ptr = c->bwd_dep_ptr.p;
-    bwd.dep = c->bwd_dep.p;
     const int grid = c->fp.mixed ? c->sweep_grid_mixed : c->sweep_grid;
     launch_sweeps_async(g, s, c->fp, fwd, bwd, std::max(1, grid / std::max(1, c->shapes.sweep_div)), c->stream);
     return launch_error("implicit sweeps");
@@ -835,42 +828,32 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
         const Schedule& S = c->sched;
         const size_t m = S.order.size();
         std::vector<int> seq_f(m), seq_b(m);
-        std::vector<long long> ptr_f(m + 1, 0), ptr_b(m + 1, 0);
-        std::vector<int> dep_f, dep_b;
-        dep_f.reserve(S.fwd_dep.size());
-        dep_b.reserve(S.bwd_dep.size());
-        // dependencies of each point: sweep position t of the forward sweep
-        // is order[t], of the backward sweep order[m-1-t]
+        // sweep position t of the forward sweep is order[t], of the backward
+        // sweep order[m-1-t]
         std::vector<int> at(n, -1);
         for (size_t t = 0; t < m; ++t) at[S.order[t]] = (int)t;
+        // The sweep kernel waits, entry by entry, on exactly the dependencies
+        // the schedule lists; each must come earlier in the execution order,
+        // or a wait would never end.  Checked here, on the host.
         auto build = [&](const std::vector<int>& lev_points, const std::vector<long long>& dptr,
-                         const std::vector<int>& dep, bool forward, std::vector<int>& seq,
-                         std::vector<long long>& ptr, std::vector<int>& out) -> int {
+                         const std::vector<int>& dep, bool forward, std::vector<int>& seq) -> int {
             std::vector<int> xpos(n, -1);
             for (size_t u = 0; u < m; ++u) xpos[lev_points[u]] = (int)u;
             for (size_t u = 0; u < m; ++u) {
                 const int p = lev_points[u];
                 seq[u] = inv[p];
                 const size_t t = forward ? (size_t)at[p] : m - 1 - (size_t)at[p];
-                for (long long e = dptr[t]; e < dptr[t + 1]; ++e) {
-                    // the sync-free sweeps wait on nothing but earlier positions
+                for (long long e = dptr[t]; e < dptr[t + 1]; ++e)
                     if (xpos[dep[e]] < 0 || xpos[dep[e]] >= (int)u) return -1;
-                    out.push_back(inv[dep[e]]);
-                }
-                ptr[u + 1] = (long long)out.size();
             }
             return 0;
         };
-        if (build(S.fwd_points, S.fwd_dep_ptr, S.fwd_dep, true, seq_f, ptr_f, dep_f))
+        if (build(S.fwd_points, S.fwd_dep_ptr, S.fwd_dep, true, seq_f))
             return fail(MFCFD_EINVAL, "forward sweep dependency out of order");
-        if (build(S.bwd_points, S.bwd_dep_ptr, S.bwd_dep, false, seq_b, ptr_b, dep_b))
+        if (build(S.bwd_points, S.bwd_dep_ptr, S.bwd_dep, false, seq_b))
             return fail(MFCFD_EINVAL, "backward sweep dependency out of order");
         if ((rc = upload(c->fwd_seq, seq_f.data(), m, st))) return rc;
         if ((rc = upload(c->bwd_seq, seq_b.data(), m, st))) return rc;
-        if ((rc = upload(c->fwd_dep, dep_f.data(), dep_f.size(), st))) return rc;
-        if ((rc = upload(c->bwd_dep, dep_b.data(), dep_b.size(), st))) return rc;
-        if ((rc = upload(c->fwd_dep_ptr, ptr_f.data(), ptr_f.size(), st))) return rc;
-        if ((rc = upload(c->bwd_dep_ptr, ptr_b.data(), ptr_b.size(), st))) return rc;
         if (c->sweep_ctr.alloc(2) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep counters");
         HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, 2 * sizeof(unsigned), st));
         HIP_TRY(hipMemsetAsync(c->rec.p, 0, sizeof(ResidualRecord), st));

@@ -47,9 +47,7 @@ struct SweepSchedule {
 
 // sync-free sweeps (kernels_sweep.hip); one per sweep direction
 struct SweepAsync {
-    const int* seq;             // swept points (device numbers) in sweep order
-    const long long* dep_ptr;   // [count+1]: dependencies of position t are dep[dep_ptr[t] .. dep_ptr[t+1])
-    const int* dep;             // device point numbers
+    const int* seq;             // swept points (device numbers) in execution order
     int count;
     unsigned* ctr;              // [2] ticket counter, finished workgroups (zero between sweeps)
     int* stalls;                // waits that hit the spin limit

@@ -121,9 +121,9 @@ struct mfcfd_ctx {
     // geometry, device order
     DevBuf<double4> pos;
     DevBuf<double> frame, coef;
-    DevBuf<int> status, nbhs, conn, dir_cnt, lst, alias, respos, perm, walk, nwalk;
+    DevBuf<int> status, nbhs, conn, dir_cnt, lst, alias, respos, perm, nwalk;
     DevBuf<long long> dir_off;
-    DevBuf<unsigned char> mask;
+    DevBuf<unsigned> walk;
     DevBuf<int> idx_interior, idx_wall, idx_outer, idx_sup_in, idx_sup_out;
     DevBuf<int> idx_fused_interior, idx_fused_wall, idx_generic;
     TypeLists tl{};
@@ -184,7 +184,6 @@ struct mfcfd_ctx {
         g.lst = lst.p;
         g.walk = walk.p;
         g.nwalk = nwalk.p;
-        g.mask = mask.p;
         g.coef = coef.p;
         g.alias = alias.p;
         g.respos = respos.p;
@@ -215,7 +214,7 @@ struct mfcfd_ctx {
         DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &q, &qm, &dq, &delt, &tsden, &flux_res, &delUp, &delUn,
                                  &dinv, &temp, &staging, &partial, &hist};
         for (auto* b : dbl) b->release();
-        DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &walk, &nwalk,
+        DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &nwalk,
                                &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
                                &idx_fused_wall, &idx_generic, &partial_pos, &fwd_points, &bwd_points,
@@ -230,7 +229,7 @@ struct mfcfd_ctx {
         DevBuf<double>* db[] = {&inst_fs, &inst_norm, &ressq, &inst_hist};
         for (auto* b : db) b->release();
         dir_off.release();
-        mask.release();
+        walk.release();
         pos.release();
     }
     ~mfcfd_ctx() {
@@ -562,8 +561,9 @@ namespace {
 int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_inst, const double* fs) {
     int rc = 0;
     if (!gm || gm->n <= 0) return fail(MFCFD_EINVAL, "empty geometry");
-    // gather kernels run 4 lanes per point; thread indices are 32-bit
-    if (gm->n > (1 << 28)) return fail(MFCFD_EINVAL, "%d points: at most 2^28 per context", gm->n);
+    // walk entries pack a point number in 26 bits (and gather kernels run
+    // 4 lanes per point on 32-bit thread indices)
+    if (gm->n > kMaxPoints) return fail(MFCFD_EINVAL, "%d points: at most %d per context", gm->n, kMaxPoints);
     const int n = gm->n;
     const double* dp[] = {gm->x, gm->y, gm->z, gm->tan1, gm->tan2, gm->nor, gm->min_dist};
     for (auto* p : dp)
@@ -643,7 +643,8 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     // conn with each entry in exactly one list per axis take the fused
     // kernel's interior class; the others, with the wall and outer points,
     // its general class.
-    std::vector<int> walk((size_t)kWalkCols * n, 0), nwalk(n, 0);
+    std::vector<unsigned> walk((size_t)kWalkCols * n, 0);
+    std::vector<int> nwalk(n, 0);
     std::vector<unsigned char> mask((size_t)kWalkCols * n, 0);
     std::vector<unsigned char> walkable(n, 0);
     c->n_regular = 0;
@@ -654,9 +655,9 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
         bool reg = true;
         const int nb = nbhs[d];
         const int* cr = gm->conn + 27LL * r;
-        int* wk = walk.data() + (size_t)kWalkCols * d;
+        unsigned* wk = walk.data() + (size_t)kWalkCols * d;
         unsigned char* mk = mask.data() + (size_t)kWalkCols * d;
-        for (int j = 0; j < nb; ++j) wk[j] = conn0[27ULL * d + j];
+        for (int j = 0; j < nb; ++j) wk[j] = (unsigned)conn0[27ULL * d + j];
         int nw = nb;
         for (int dir = 0; dir < 6; ++dir) {
             const HostList hl = host_list(gm, r, dir);
@@ -673,7 +674,7 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
                     ++j;
                 } else if (nw < kWalkCols) {
                     reg = false;
-                    wk[nw] = inv[k1 - 1];
+                    wk[nw] = (unsigned)inv[k1 - 1];
                     mk[nw] = (unsigned char)(1u << dir);
                     ++nw;
                 } else {
@@ -688,6 +689,7 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
                 const unsigned ab = (mk[j] >> (2 * a)) & 3u;
                 if (!(ab == 1u || ab == 2u)) simple = false;
             }
+        for (int j = 0; j < nw; ++j) wk[j] = (wk[j] << kWalkMaskBits) | mk[j];
         nwalk[d] = nw;
         walkable[d] = fits ? (simple ? 2 : 1) : 0;
         c->n_regular += reg ? 1 : 0;
@@ -730,7 +732,6 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     if ((rc = upload(c->dir_cnt, cnt.data(), cnt.size(), st))) return rc;
     if ((rc = upload(c->dir_off, off.data(), off.size(), st))) return rc;
     if ((rc = upload(c->lst, lst.data(), lst.size(), st))) return rc;
-    if ((rc = upload(c->mask, mask.data(), mask.size(), st))) return rc;
     if ((rc = upload(c->walk, walk.data(), walk.size(), st))) return rc;
     if ((rc = upload(c->nwalk, nwalk.data(), nwalk.size(), st))) return rc;
     if ((rc = upload(c->alias, alias.data(), alias.size(), st))) return rc;

@@ -35,6 +35,10 @@ constexpr int kConnCols = 27;
 // entries per point of the fused flux kernel's walk: conn, then the list
 // entries that are not an in-order subsequence of conn (22-column spill)
 constexpr int kWalkCols = 32;
+// a walk entry packs the neighbour's device number above kWalkMaskBits bits
+// of direction-list membership (bit d: the entry is in list d)
+constexpr int kWalkMaskBits = 6;
+constexpr int kMaxPoints = 1 << (32 - kWalkMaskBits);
 constexpr int kSplitCols = 22;
 constexpr int kSplitSlack = 32;
 
@@ -50,9 +54,8 @@ struct DevGeom {
     const int* dir_cnt;          // [6][n]
     const long long* dir_off;    // [6][n]
     const int* lst;
-    const int* walk;             // [kWalkCols n] entries the fused flux kernel visits
+    const unsigned* walk;        // [kWalkCols n] entries the fused flux kernel visits, packed
     const int* nwalk;            // [n] used entries of walk
-    const unsigned char* mask;   // [kWalkCols n] direction lists each walk entry belongs to
     const double* coef;          // [18n]
     const int* alias;
     const int* respos;           // position in the residual order (wall, then interior)

@@ -249,15 +249,15 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_flux_fused(DevGeom g, DevStat
     const T epsi2_i = T(limiter_epsi2(fp.vl_const, pi4.w));
     const double* cf = g.coef + 18 * (long long)i;
     const int nb = g.nwalk[i];
-    const int* cn = g.walk + kWalkCols * (long long)i;
-    const unsigned char* mk = g.mask + kWalkCols * (long long)i;
+    const unsigned* wk = g.walk + kWalkCols * (long long)i;
     const int st_i = CLASS == 1 ? g.status[i] : 0;
     const T sn = st_i == 2 ? T(1) : T(-1);
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int j = lane; j < nb; j += kFluxGroup) {
-        const unsigned bits = mk[j];
+        const unsigned e = wk[j];
+        const unsigned bits = e & ((1u << kWalkMaskBits) - 1u);
         if (bits == 0u) continue;
-        const int k = cn[j];
+        const int k = (int)(e >> kWalkMaskBits);
         const double4 pk = g.pos[k];
         const double delx = pk.x - pi4.x;
         const double dely = pk.y - pi4.y;

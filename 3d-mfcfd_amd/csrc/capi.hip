@@ -799,10 +799,10 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
         return fail(MFCFD_ENOMEM, "state arrays");
     HIP_TRY(hipMemsetAsync(c->qm.p, 0, 10ULL * n * sizeof(double), st));
     HIP_TRY(hipMemsetAsync(c->dq.p, 0, 15ULL * n * sizeof(double), st));
-    if (c->q32.alloc(5ULL * n) != hipSuccess || c->qm32.alloc(10ULL * n) != hipSuccess ||
+    if (c->q32.alloc((size_t)kQ32Stride * n) != hipSuccess || c->qm32.alloc(10ULL * n) != hipSuccess ||
         c->dq32.alloc(15ULL * n) != hipSuccess)
         return fail(MFCFD_ENOMEM, "FP32 q arrays");
-    HIP_TRY(hipMemsetAsync(c->q32.p, 0, 5ULL * n * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->q32.p, 0, (size_t)kQ32Stride * n * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->qm32.p, 0, 10ULL * n * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->dq32.p, 0, 15ULL * n * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->delt.p, 0, (size_t)n * sizeof(double), st));
@@ -1066,7 +1066,7 @@ int mfcfd_upload(mfcfd_ctx* c, int id, const double* host) {
     launch_gather(a.buf->p, c->staging.p, c->perm.p, c->n, a.per_point, c->stream);
     // mixed form: the double array is staging for the FP32 copy the kernels read
     if (float* f32 = c->fp.mixed ? mixed_copy_of(c, id) : nullptr)
-        launch_to_float(f32, a.buf->p, (long long)a.per_point * c->n, c->stream);
+        launch_to_float(f32, id == MFCFD_ARR_Q ? kQ32Stride : a.per_point, a.buf->p, a.per_point, c->n, c->stream);
     if ((rc = launch_error("upload permutation"))) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MFCFD_OK;
@@ -1079,7 +1079,7 @@ int mfcfd_download(mfcfd_ctx* c, int id, double* host) {
     if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
     const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
     if (float* f32 = c->fp.mixed ? mixed_copy_of(c, id) : nullptr)
-        launch_to_double(a.buf->p, f32, (long long)a.per_point * c->n, c->stream);
+        launch_to_double(a.buf->p, f32, id == MFCFD_ARR_Q ? kQ32Stride : a.per_point, a.per_point, c->n, c->stream);
     launch_scatter(c->staging.p, a.buf->p, c->perm.p, c->n, a.per_point, c->stream);
     if ((rc = launch_error("download permutation"))) return rc;
     HIP_TRY(hipMemcpyAsync(host, c->staging.p, bytes, hipMemcpyDeviceToHost, c->stream));

@@ -78,11 +78,16 @@ struct DevState {
     double* delUp;
     double* delUn;
     double* dinv;
-    // mixed precision: q, qm, dq held in FP32 (same layouts)
+    // mixed precision: q, qm, dq held in FP32 (the reference layouts, except
+    // that q32 holds kQ32Stride floats per point: q, then the time step's
+    // denominator tsden in FP32, which the q-derivative kernel then reads
+    // with the same two loads as q)
     float* q32;
     float* qm32;
     float* dq32;
 };
+
+constexpr int kQ32Stride = 6;
 
 // q-variable arrays of one precision: point k's q (5), qm (2,5) and dq (3,5)
 // start at q + sq*k, qm + sqm*k, dq + sdq*k
@@ -105,7 +110,7 @@ __host__ __device__ inline QArrays<double> qarrays<double>(const DevState& s) {
 }
 template <>
 __host__ __device__ inline QArrays<float> qarrays<float>(const DevState& s) {
-    return {s.q32, s.qm32, s.dq32, 5, 10, 15};
+    return {s.q32, s.qm32, s.dq32, kQ32Stride, 10, 15};
 }
 
 struct FlowParams {

@@ -58,8 +58,9 @@ void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, h
 // boundary permutations: dst[k*m + c] = src[perm[k]*m + c] and the inverse
 void launch_gather(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
 // precision conversions of a flat array (mixed form uploads and downloads)
-void launch_to_float(float* dst, const double* src, long long n, hipStream_t st);
-void launch_to_double(double* dst, const float* src, long long n, hipStream_t st);
+// (m values per point; the FP32 array holds stride32 per point)
+void launch_to_float(float* dst, int stride32, const double* src, int m, long long n, hipStream_t st);
+void launch_to_double(double* dst, const float* src, int stride32, int m, long long n, hipStream_t st);
 void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
 
 // compute_conserved_vector.f90 / q_variables.f90

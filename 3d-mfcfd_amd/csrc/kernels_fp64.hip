@@ -119,7 +119,9 @@ __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) qo[r] = QT(q[r]);
         // timestep_delt.f90:40-42: the neighbour-only part of the local step
-        s.tsden[k] = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3) + 3.0 * __builtin_sqrt(pr / rho);
+        const double den = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3) + 3.0 * __builtin_sqrt(pr / rho);
+        s.tsden[k] = den;
+        if (sizeof(QT) == 4) qo[5] = QT(den);
     }
 }
 
@@ -201,7 +203,9 @@ __global__ void __launch_bounds__(kBlock, WAVES) k_qderiv_timestep(DevGeom g, De
         if (DO_DT) {
             // timestep_delt.f90:34-47, the neighbour's denominator read, not recomputed
             const double dist = __builtin_sqrt(delx * delx + dely * dely + delz * delz);
-            const double delt = fp.cfl * dist / s.tsden[k];
+            // the mixed form reads its FP32 copy, next to q
+            const double den = sizeof(QT) == 4 && DO_DQ ? (double)qa.qp(k)[5] : s.tsden[k];
+            const double delt = fp.cfl * dist / den;
             if (delt < min_delt) min_delt = delt;
         }
     }
@@ -284,14 +288,20 @@ __global__ void __launch_bounds__(kBlock) k_qderiv_inner(DevGeom g, DevState s, 
             temp[(5 * (long long)i + r) * 3 + c] = cramer_num(geo, c, sx[r], sy[r], sz[r]) * one_by_det;
 }
 
-__global__ void k_to_float(float* dst, const double* src, long long n) {
-    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (float)src[i];
+// m values per point between an FP64 array of m per point and an FP32 array
+// of stride32 >= m per point
+__global__ void k_to_float(float* dst, int stride32, const double* src, int m, long long n) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const long long k = e / m;
+    dst[stride32 * k + (e - k * m)] = (float)src[e];
 }
 
-__global__ void k_to_double(double* dst, const float* src, long long n) {
-    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (double)src[i];
+__global__ void k_to_double(double* dst, const float* src, int stride32, int m, long long n) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const long long k = e / m;
+    dst[e] = (double)src[stride32 * k + (e - k * m)];
 }
 
 __global__ void k_copy(double* dst, const double* src, long long n) {
@@ -668,10 +678,11 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
                                                          double* partial, int* partial_pos,
                                                          int* partial_pt, double* partial_max,
                                                          double* ressq) {
-    __shared__ double ssum[kBlock];
-    __shared__ double smax[kBlock];
-    __shared__ int spos[kBlock];
-    __shared__ int spt[kBlock];
+    constexpr int kWaves = kBlock / 64;
+    __shared__ double ssum[kWaves];
+    __shared__ double smax[kWaves];
+    __shared__ int spos[kWaves];
+    __shared__ int spt[kWaves];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double res_sqr = 0.0;
     MaxPair mp = {0.0, 0x7fffffff, 0};
@@ -722,28 +733,35 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
         }
     }
     if (ressq && k < g.n) ressq[k] = res_sqr;
-    ssum[threadIdx.x] = res_sqr;
-    smax[threadIdx.x] = mp.v;
-    spos[threadIdx.x] = mp.pos;
-    spt[threadIdx.x] = mp.point;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + w];
-            MaxPair a = {smax[threadIdx.x], spos[threadIdx.x], spt[threadIdx.x]};
-            MaxPair b = {smax[threadIdx.x + w], spos[threadIdx.x + w], spt[threadIdx.x + w]};
-            a = max_pair(a, b);
-            smax[threadIdx.x] = a.v;
-            spos[threadIdx.x] = a.pos;
-            spt[threadIdx.x] = a.point;
-        }
-        __syncthreads();
+    // block partials: a butterfly over each wave (every lane ends with the
+    // same, order-fixed sum), then the waves' results in wave order
+    double sum = res_sqr;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        sum = sum + __shfl_xor(sum, o, 64);
+        const MaxPair b = {__shfl_xor(mp.v, o, 64), __shfl_xor(mp.pos, o, 64), __shfl_xor(mp.point, o, 64)};
+        mp = max_pair(mp, b);
     }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        ssum[wave] = sum;
+        smax[wave] = mp.v;
+        spos[wave] = mp.pos;
+        spt[wave] = mp.point;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        partial[blockIdx.x] = ssum[0];
-        partial_max[blockIdx.x] = smax[0];
-        partial_pos[blockIdx.x] = spos[0];
-        partial_pt[blockIdx.x] = spt[0];
+        double total = ssum[0];
+        MaxPair best = {smax[0], spos[0], spt[0]};
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) {
+            total = total + ssum[w];
+            best = max_pair(best, MaxPair{smax[w], spos[w], spt[w]});
+        }
+        partial[blockIdx.x] = total;
+        partial_max[blockIdx.x] = best.v;
+        partial_pos[blockIdx.x] = best.pos;
+        partial_pt[blockIdx.x] = best.point;
     }
 }
 
@@ -867,13 +885,13 @@ void launch_gather(double* dst, const double* src, const int* perm, int n, int m
     hipLaunchKernelGGL(k_gather, dim3(grid_for((long long)n * m, kBlock)), dim3(kBlock), 0, st, dst, src, perm,
                        n, m);
 }
-void launch_to_float(float* dst, const double* src, long long n, hipStream_t st) {
+void launch_to_float(float* dst, int stride32, const double* src, int m, long long n, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_to_float, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, dst, src, n);
+    hipLaunchKernelGGL(k_to_float, dim3(grid_for(m * n, kBlock)), dim3(kBlock), 0, st, dst, stride32, src, m, n);
 }
-void launch_to_double(double* dst, const float* src, long long n, hipStream_t st) {
+void launch_to_double(double* dst, const float* src, int stride32, int m, long long n, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_to_double, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, dst, src, n);
+    hipLaunchKernelGGL(k_to_double, dim3(grid_for(m * n, kBlock)), dim3(kBlock), 0, st, dst, src, stride32, m, n);
 }
 void launch_scatter(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st) {
     if (n <= 0) return;

@@ -3,8 +3,9 @@
 // A context owns the HBM copies of the point cloud and the flow state.  At
 // bind time the points are renumbered in Morton order (device order), each
 // point's six direction lists are resolved from the host's (22,N) arrays
-// exactly as the reference reads them, regular points are marked for the
-// fused flux kernel, and the Cramer weights of every list are computed.
+// exactly as the reference reads them, each point gets the walk the fused
+// flux kernel visits (conn plus the entries the 22-column spill adds), and
+// the Cramer weights of every list are computed.
 // Entry points enqueue kernels on the context's stream and return; only
 // fpi_solver/run/state_update/get_residual/download wait for the device.
 #include <hip/hip_runtime.h>
@@ -138,7 +139,7 @@ struct mfcfd_ctx {
     TypeLists tl{};
     // state
     DevBuf<double> prim, U, q, qm, dq, delt, tsden, flux_res, delUp, delUn, dinv, temp, staging;
-    DevBuf<float> q32, qm32, dq32;   // mixed precision
+    DevBuf<float> q32, qm32, dq32;   // mixed precision (q32: q and an FP32 tsden per point)
     // residual
     DevBuf<double> partial;
     DevBuf<int> partial_pos;

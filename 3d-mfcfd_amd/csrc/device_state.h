@@ -18,8 +18,11 @@
 //                     the next point's column).  Wall points store their full
 //                     conn as zneg (wall_flux_dGzneg.f90:61-65), outer points
 //                     their zpos list; lst[dir_off .. dir_off + dir_cnt) holds it
-//   mask[27k + j]     for regular points (every list an in-order subsequence of
-//                     conn): bit d set when conn entry j belongs to list d
+//   conn[27k + j]     (as above) and walk[32k + j], nwalk[k]: the entries the
+//                     fused flux kernel visits -- conn, then every list entry
+//                     that does not continue an in-order match against it --
+//                     each packing the neighbour's number above six bits,
+//                     bit d set when the entry belongs to list d
 //   coef[18k + 3d ..] Cramer weights of list d: the derivative of that stencil is
 //                     sum_j (G_k - G_i) * (c0*dels_j + c1*delt_j + c2*deln_j)
 //   prim/U/q/flux_res/delUp/delUn [5k + r], qm [(5k + r)*2 + s],

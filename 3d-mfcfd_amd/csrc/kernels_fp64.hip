@@ -1,12 +1,14 @@
 // kernels_fp64.hip -- FP64 kernels of the q-LSKUM pseudo-time iteration in
 // the reference's evaluation order.
 //
-// One thread per point; points of one class are launched together.  Every
-// list-walking routine below follows the reference routine named at it with
-// the same association of every sum and product; the file is compiled with
-// -ffp-contract=off.  The fused flux kernels (kernels_flux_fused.hip) replace
-// the flux residual for regular points in the iteration; this file keeps the
-// list-by-list form for the remaining points and for the parity entry point.
+// Streaming kernels take one thread per point; the gathering kernels a group
+// of lanes per point (q-derivatives: 4 lanes splitting conn; list-by-list
+// flux: one lane per direction list).  Every list-walking routine below
+// follows the reference routine named at it with the same association of
+// each product; the file is compiled with -ffp-contract=off.  The fused flux
+// kernels (kernels_flux_fused.hip) give the flux residual of every point
+// whose walk fits in 32 entries; this file keeps the list-by-list form for
+// the rest and for the reference-order mode.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"

@@ -151,8 +151,11 @@ struct mfcfd_ctx {
     Schedule sched;
     std::vector<int> fwd_ptr, bwd_ptr;
     DevBuf<int> fwd_points, bwd_points;
-    // sync-free sweeps: points in sweep order, dependency lists, counters
+    // sync-free sweeps: points in execution order, the list entries each
+    // position takes, counters
     DevBuf<int> fwd_seq, bwd_seq;
+    DevBuf<long long> fwd_take_ptr, bwd_take_ptr;
+    DevBuf<unsigned> fwd_take, bwd_take;
     DevBuf<unsigned> sweep_ctr;
     // clouds of a batch (mfcfd_bind_batch): instance of each device point,
     // freestream (u1_inf, u2_inf) per instance, device-order ranges; a single
@@ -242,6 +245,10 @@ struct mfcfd_ctx {
         dir_off.release();
         walk.release();
         pos.release();
+        fwd_take_ptr.release();
+        bwd_take_ptr.release();
+        fwd_take.release();
+        bwd_take.release();
     }
     ~mfcfd_ctx() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -375,6 +382,10 @@ int do_sweeps(mfcfd_ctx* c) {
     fwd.stalls = bwd.stalls = &c->rec.p->stalls;
     fwd.seq = c->fwd_seq.p;
     bwd.seq = c->bwd_seq.p;
+    fwd.take_ptr = c->fwd_take_ptr.p;
+    bwd.take_ptr = c->bwd_take_ptr.p;
+    fwd.take = c->fwd_take.p;
+    bwd.take = c->bwd_take.p;
     const int grid = c->fp.mixed ? c->sweep_grid_mixed : c->sweep_grid;
     launch_sweeps_async(g, s, c->fp, fwd, bwd, std::max(1, grid / std::max(1, c->shapes.sweep_div)), c->stream);
     return launch_error("implicit sweeps");
@@ -873,6 +884,38 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
             return fail(MFCFD_EINVAL, "backward sweep dependency out of order");
         if ((rc = upload(c->fwd_seq, seq_f.data(), m, st))) return rc;
         if ((rc = upload(c->bwd_seq, seq_b.data(), m, st))) return rc;
+        // the entries of each position's direction lists that its sweep
+        // takes: neighbours with a smaller alias (forward) or a larger one
+        // (backward), every list but a wall point's ZPOS (its z term walks
+        // conn, the ZNEG list), each packed as k << 3 | direction
+        auto takes = [&](const std::vector<int>& seq, bool forward, DevBuf<long long>& dptr,
+                         DevBuf<unsigned>& dent) -> int {
+            std::vector<long long> ptr(m + 1, 0);
+            std::vector<unsigned> ent;
+            ent.reserve(m * 40);
+            for (size_t u = 0; u < m; ++u) {
+                const int d = seq[u];
+                const int ad = alias[d];
+                for (int dir = 0; dir < 6; ++dir) {
+                    if (status[d] == 1 && dir == ZPOS) continue;
+                    const int cn = cnt[(size_t)dir * n + d];
+                    const long long o = off[(size_t)dir * n + d];
+                    for (int j = 0; j < cn; ++j) {
+                        const int k = lst[o + j];
+                        if (forward ? alias[k] < ad : alias[k] > ad) ent.push_back(((unsigned)k << 3) | (unsigned)dir);
+                    }
+                }
+                ptr[u + 1] = (long long)ent.size();
+            }
+            int r2 = upload(dptr, ptr.data(), ptr.size(), st);
+            if (r2) return r2;
+            if (ent.empty()) ent.push_back(0u);  // keep the buffer allocated
+            if ((r2 = upload(dent, ent.data(), ent.size(), st))) return r2;
+            HIP_TRY(hipStreamSynchronize(st));  // the host vectors end here
+            return MFCFD_OK;
+        };
+        if ((rc = takes(seq_f, true, c->fwd_take_ptr, c->fwd_take))) return rc;
+        if ((rc = takes(seq_b, false, c->bwd_take_ptr, c->bwd_take))) return rc;
         if (c->sweep_ctr.alloc(2) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep counters");
         HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, 2 * sizeof(unsigned), st));
         HIP_TRY(hipMemsetAsync(c->rec.p, 0, sizeof(ResidualRecord), st));

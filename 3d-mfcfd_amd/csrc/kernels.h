@@ -49,6 +49,8 @@ struct SweepSchedule {
 struct SweepAsync {
     const int* seq;             // swept points (device numbers) in execution order
     int count;
+    const long long* take_ptr;  // [count + 1]: position t takes take[take_ptr[t] .. take_ptr[t+1])
+    const unsigned* take;       // neighbour k << 3 | direction list of every taken entry
     unsigned* ctr;              // [2] ticket counter, finished workgroups (zero between sweeps)
     int* stalls;                // waits that hit the spin limit
 };

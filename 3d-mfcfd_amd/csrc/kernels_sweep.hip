@@ -27,11 +27,14 @@
 // limit is counted in the residual record and the host reports an error;
 // the kernel still runs to completion.
 //
-// One workgroup of 128 lanes handles one point: lane e takes entry e of the
-// point's direction lists flattened in direction order (6 lists for an
-// interior point, x/y lists and the z list over conn for a wall point), and
-// the sum over all lists (the sweep term is the sum of the directions'
-// dG, times 2 at wall points) is a fixed-order tree reduction.  Each entry's
+// One wave of 64 lanes handles one point: the host lists, per execution
+// position and sweep direction, the entries of the point's direction lists
+// that this sweep takes (the neighbours with a smaller alias in the forward
+// sweep, a larger one in the backward sweep; a wall point's z term walks
+// conn, so its ZPOS list is skipped), each packed as k << 3 | direction.
+// Lane e takes entry e (e + 64, ... for longer lists), and the sum over all
+// entries (the sweep term is the sum of the directions' dG, times 2 at wall
+// points) is a fixed-order butterfly within the wave.  Each entry's
 // contribution is (Gnew - Gold) * w (c0*dels + c1*delt + c2*deln) with the
 // Cramer cofactors c of the list (k_stencil_coef), which equals the
 // reference's Cramer solve on the summed products up to rounding.  T is the
@@ -48,8 +51,7 @@ namespace mfcfd {
 
 namespace {
 
-constexpr int kSweepBlock = 128;
-constexpr int kSweepWaves = kSweepBlock / 64;
+constexpr int kSweepBlock = 64;
 // about a second of polling; legitimate waits end within milliseconds
 constexpr int kSpinLimit = 1 << 20;
 
@@ -99,121 +101,76 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <bool FORWARD, bool UW, typename T>
 __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState s, FlowParams fp,
                                                              SweepAsync a) {
-    __shared__ unsigned sh_ticket;
-    __shared__ double sh_part[kSweepWaves][5];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = threadIdx.x;
     const double* dU = FORWARD ? s.delUp : s.delUn;
     for (;;) {
-        if (tid == 0)
-            sh_ticket = __hip_atomic_fetch_add(a.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const unsigned t = sh_ticket;
+        unsigned t = 0;
+        if (lane == 0) t = __hip_atomic_fetch_add(a.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = __shfl(t, 0, kSweepBlock);
         if (t >= (unsigned)a.count) break;
         const int i = a.seq[t];
         const int cls = g.status[i];
-        const int alias_i = g.alias[i];
         const Frame<double> f = load_frame(g.frame, i);
         const Frame<T> ft = frame_as<T>(f);
         const double4 pi4 = g.pos[i];
-        int cnt[6];
-        long long off[6];
-        int total = 0;
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            cnt[d] = g.dir_cnt[(long long)d * g.n + i];
-            off[d] = g.dir_off[(long long)d * g.n + i];
-        }
-        if (cls == 1) cnt[ZPOS] = 0;  // a wall point's z term walks conn (the ZNEG list)
-#pragma unroll
-        for (int d = 0; d < 6; ++d) total += cnt[d];
         // the point's own inputs of the final update, read before the wait
         double D_inv = 0.0, own[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (tid == 0) {
+        if (lane == 0) {
             D_inv = s.dinv[i];
 #pragma unroll
             for (int r = 0; r < 5; ++r) own[r] = FORWARD ? s.flux_res[5 * (long long)i + r] : s.delUp[5 * (long long)i + r];
         }
-
         double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int base = 0; base < total; base += kSweepBlock) {
-            const int e = base + tid;
-            bool take = false;
-            int k = 0, d = 0;
-            double wgt = 0.0, Uk[5];
-            T Gold[5];
-            Frame<double> fk;
-            if (e < total) {
-                int rem = e;
-                while (rem >= cnt[d]) {
-                    rem -= cnt[d];
-                    ++d;
-                }
-                k = g.lst[off[d] + rem];
-                const int alias_k = g.alias[k];
-                take = FORWARD ? (alias_k < alias_i) : (alias_k > alias_i);
-                if (take) {
-                    const double4 pk = g.pos[k];
-                    const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
-                    const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
-                    const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
-                    const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
-                    const double* c = g.coef + 18 * (long long)i + 3 * d;
-                    wgt = wls_weight<UW>(dels, delt, deln, fp.power) * (c[0] * dels + c[1] * delt + c[2] * deln);
-                    T prim[5];
+        const long long tb = a.take_ptr[t], te = a.take_ptr[t + 1];
+        for (long long e = tb + lane; e < te; e += kSweepBlock) {
+            const unsigned packed = a.take[e];
+            const int k = (int)(packed >> 3);
+            const int d = (int)(packed & 7u);
+            // everything that does not depend on this sweep's values first
+            const double4 pk = g.pos[k];
+            const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
+            const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+            const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+            const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+            const double* c = g.coef + 18 * (long long)i + 3 * d;
+            const double wgt = wls_weight<UW>(dels, delt, deln, fp.power) * (c[0] * dels + c[1] * delt + c[2] * deln);
+            T prim_old[5], Gold[5];
 #pragma unroll
-                    for (int r = 0; r < 5; ++r) prim[r] = T(s.prim[5 * (long long)k + r]);
-                    flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim, T(fp.pi), Gold);
+            for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * (long long)k + r]);
+            flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
+            double Uk[5];
 #pragma unroll
-                    for (int r = 0; r < 5; ++r) Uk[r] = s.U[5 * (long long)k + r];
-                    fk = load_frame(g.frame, k);
-                }
-            }
-            if (take) {
-                // the lanes that take an entry are exactly this point's
-                // dependencies in the schedule: each waits for its own
-                // neighbour and uses the values it polled
-                double U[5], prim[5], dk[5];
-                T pt[5], Gnew[5];
-                wait_read(dU + 5 * (long long)k, dk, a.stalls);
+            for (int r = 0; r < 5; ++r) Uk[r] = s.U[5 * (long long)k + r];
+            const Frame<double> fk = load_frame(g.frame, k);
+            // then this neighbour's values of the current sweep
+            double dk[5], U[5], prim[5];
+            wait_read(dU + 5 * (long long)k, dk, a.stalls);
 #pragma unroll
-                for (int r = 0; r < 5; ++r) U[r] = dk[r] + Uk[r];
-                conserved_to_prim(U, fk, prim);
+            for (int r = 0; r < 5; ++r) U[r] = dk[r] + Uk[r];
+            conserved_to_prim(U, fk, prim);
+            T pt[5], Gnew[5];
 #pragma unroll
-                for (int r = 0; r < 5; ++r) pt[r] = T(prim[r]);
-                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
+            for (int r = 0; r < 5; ++r) pt[r] = T(prim[r]);
+            flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
 #pragma unroll
-                for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
-            }
+            for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
         }
 #pragma unroll
         for (int r = 0; r < 5; ++r) acc[r] = wave_sum(acc[r]);
         if (lane == 0) {
-#pragma unroll
-            for (int r = 0; r < 5; ++r) sh_part[wave][r] = acc[r];
-        }
-        __syncthreads();
-        if (tid == 0) {
-            double T5[5];
-#pragma unroll
-            for (int r = 0; r < 5; ++r) {
-                double v = sh_part[0][r];
-#pragma unroll
-                for (int w = 1; w < kSweepWaves; ++w) v = v + sh_part[w][r];
-                T5[r] = cls == 1 ? 2.0 * v : v;
-            }
             if (FORWARD) {
 #pragma unroll
-                for (int r = 0; r < 5; ++r) store_value(s.delUp + 5 * (long long)i + r, -(D_inv * (own[r] + T5[r])));
+                for (int r = 0; r < 5; ++r)
+                    store_value(s.delUp + 5 * (long long)i + r, -(D_inv * (own[r] + (cls == 1 ? 2.0 * acc[r] : acc[r]))));
             } else {
 #pragma unroll
-                for (int r = 0; r < 5; ++r) store_value(s.delUn + 5 * (long long)i + r, own[r] - D_inv * T5[r]);
+                for (int r = 0; r < 5; ++r)
+                    store_value(s.delUn + 5 * (long long)i + r, own[r] - D_inv * (cls == 1 ? 2.0 * acc[r] : acc[r]));
             }
         }
-        __syncthreads();
     }
     // the last workgroup out resets the counters for the next sweep
-    if (tid == 0) {
+    if (lane == 0) {
         const unsigned done = __hip_atomic_fetch_add(a.ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (done == gridDim.x - 1) {
             __hip_atomic_store(a.ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

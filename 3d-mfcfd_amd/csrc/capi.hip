@@ -1388,7 +1388,9 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
         launch_timestep(g, s, c->fp, c->stream);
     }
     if (c->params.flux_order == MFCFD_FLUX_FUSED) {
-        const int waves[4] = {1, 2, 3, 4};
+        // FP32 variants also come capped at 5 and 6 waves per SIMD (a few
+        // spilled registers for more resident waves)
+        const std::vector<int> waves = c->fp.mixed ? std::vector<int>{1, 2, 3, 4, 5, 6} : std::vector<int>{1, 2, 3, 4};
         for (int part = 1; part <= 2; ++part) {
             const int count = part == 1 ? c->tl.n_fused_interior : c->tl.n_fused_wall;
             if (count == 0) continue;

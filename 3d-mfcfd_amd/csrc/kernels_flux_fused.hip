@@ -305,6 +305,18 @@ void launch_class(const DevGeom& g, const DevState& s, const FlowParams& p, cons
         case 1: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 1>), grid, blk, ks.lds, st, g, s, p, idx, count); break;
         case 3: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 3>), grid, blk, ks.lds, st, g, s, p, idx, count); break;
         case 4: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 4>), grid, blk, ks.lds, st, g, s, p, idx, count); break;
+        case 5:
+            if constexpr (sizeof(T) == 4) {
+                hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 5>), grid, blk, ks.lds, st, g, s, p, idx, count);
+                break;
+            }
+            [[fallthrough]];
+        case 6:
+            if constexpr (sizeof(T) == 4) {
+                hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 6>), grid, blk, ks.lds, st, g, s, p, idx, count);
+                break;
+            }
+            [[fallthrough]];
         default: hipLaunchKernelGGL((k_flux_fused<CLASS, true, T, 2>), grid, blk, ks.lds, st, g, s, p, idx, count); break;
     }
 }

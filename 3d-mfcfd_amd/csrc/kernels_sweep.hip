@@ -8,16 +8,19 @@
 // larger aliases.  The result is a Gauss-Seidel recurrence whose dependency
 // graph is fixed by the alias order.
 //
-// Here a persistent grid of workgroups takes tickets from a counter; ticket
-// t processes position t of the execution order.  Every point a workgroup
-// waits on has a smaller position, so it belongs to a workgroup that took
-// its ticket earlier and is running: the lowest unfinished ticket never
-// waits and every wait ends.
+// Here a persistent grid of workgroups takes tickets from a counter: each
+// round a workgroup of kSweepWaves waves takes kSweepWaves consecutive
+// positions with one atomic, and wave w processes position base + w of the
+// execution order.  Every point a wave waits on has a smaller position, so
+// it was taken earlier (or in the same round) by a wave that is running.
+// The lowest unfinished position never waits, and the barrier between a
+// workgroup's rounds only delays its next positions, which are higher than
+// every position taken, so every wait ends.
 //
 // The values themselves are the completion flags.  Before a sweep, a kernel
 // marks the five outputs of every swept point with a signalling-NaN
 // pattern that no arithmetic produces (kPending).  A point writes its five
-// results once, with agent-scope relaxed atomic stores; a workgroup first
+// results once, with agent-scope relaxed atomic stores; a wave first
 // computes everything that does not depend on this sweep's values
 // (geometry, Cramer weights, the old fluxes Gold), then polls its
 // dependencies' values with agent-scope relaxed atomic loads until none is
@@ -27,7 +30,7 @@
 // limit is counted in the residual record and the host reports an error;
 // the kernel still runs to completion.
 //
-// One wave of 64 lanes handles one point: the host lists, per execution
+// One wave of 64 lanes handles one point.  The host lists, per execution
 // position and sweep direction, the entries of the point's direction lists
 // that this sweep takes (the neighbours with a smaller alias in the forward
 // sweep, a larger one in the backward sweep; a wall point's z term walks
@@ -51,7 +54,12 @@ namespace mfcfd {
 
 namespace {
 
-constexpr int kSweepBlock = 64;
+// waves per workgroup: one point per wave, and one ticket counter atomic per
+// workgroup and round for kSweepWaves consecutive positions (same-address
+// device-scope atomics serialise: ~12 ns each, which caps single-ticket
+// rounds at ~80 points per microsecond)
+constexpr int kSweepWaves = 4;
+constexpr int kSweepBlock = 64 * kSweepWaves;
 // about a second of polling; legitimate waits end within milliseconds
 constexpr int kSpinLimit = 1 << 20;
 
@@ -101,13 +109,22 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <bool FORWARD, bool UW, typename T>
 __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState s, FlowParams fp,
                                                              SweepAsync a) {
-    const int lane = threadIdx.x;
+    // double-buffered: round r writes slot r & 1 while a wave may still read
+    // the other; a slot is rewritten two rounds later, after a barrier that
+    // every wave reaches only after reading it
+    __shared__ unsigned sh_base[2];
+    const int lane = threadIdx.x & 63;
+    const unsigned wave = threadIdx.x >> 6;
     const double* dU = FORWARD ? s.delUp : s.delUn;
-    for (;;) {
-        unsigned t = 0;
-        if (lane == 0) t = __hip_atomic_fetch_add(a.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t = __shfl(t, 0, kSweepBlock);
-        if (t >= (unsigned)a.count) break;
+    for (unsigned round = 0;; ++round) {
+        if (threadIdx.x == 0)
+            sh_base[round & 1] = __hip_atomic_fetch_add(a.ctr, (unsigned)kSweepWaves, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const unsigned base = sh_base[round & 1];
+        if (base >= (unsigned)a.count) break;   // the same for the whole workgroup
+        const unsigned t = base + wave;
+        if (t >= (unsigned)a.count) continue;   // the last, partial round
         const int i = a.seq[t];
         const int cls = g.status[i];
         const Frame<double> f = load_frame(g.frame, i);
@@ -122,7 +139,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         }
         double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         const long long tb = a.take_ptr[t], te = a.take_ptr[t + 1];
-        for (long long e = tb + lane; e < te; e += kSweepBlock) {
+        for (long long e = tb + lane; e < te; e += 64) {
             const unsigned packed = a.take[e];
             const int k = (int)(packed >> 3);
             const int d = (int)(packed & 7u);
@@ -170,7 +187,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         }
     }
     // the last workgroup out resets the counters for the next sweep
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         const unsigned done = __hip_atomic_fetch_add(a.ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (done == gridDim.x - 1) {
             __hip_atomic_store(a.ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

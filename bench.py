@@ -116,9 +116,10 @@ def bench_batch(args):
     for precision in (["fp64"] if args.no_mixed else ["fp64", "mixed"]):
         bs = solver.BatchSolver(clouds, machs, aoas, cfg["cfl"], mode=cfg["mode"], precision=precision)
         bs.upload("prim", p0)
-        tuning = None if args.no_tune else bs.autotune()
         if W > 0:
             bs.run_graph(1, W)
+        # shapes tuned on the state the timed iterations start from
+        tuning = None if args.no_tune else bs.autotune()
         bs.synchronize()
         t0 = time.perf_counter()
         _, inst = bs.run_graph(W + 1, K)
@@ -218,17 +219,17 @@ def main():
                       split=(cnt, lists), alias=alias)
     s.upload("prim", prim0)
     log(f"bound; sweep schedule {s.sweep_info()}")
-    tuning = None
-    if not args.no_tune:
-        log("autotune")
-        tuning = s.autotune()
-        log(f"autotune: {tuning}")
-
     W, K = args.warmup, args.steps
     log("warmup")
     if W > 0:
         s.run(1, W)
     s.synchronize()
+    # shapes tuned on the state the timed iterations start from
+    tuning = None
+    if not args.no_tune:
+        log("autotune")
+        tuning = s.autotune()
+        log(f"autotune: {tuning}")
     log(f"warmup {W} done; timed iterations")
     s.kernel_timing(True)
     t0 = time.perf_counter()
@@ -282,12 +283,12 @@ def main():
         m = solver.Solver(cloud, cfg["mach"], cfg["aoa"], cfg["cfl"], mode=cfg["mode"],
                           precision="mixed", split=(cnt, lists), alias=alias)
         m.upload("prim", prim0)
-        log("mixed leg: autotune")
-        tuning_m = m.autotune() if not args.no_tune else None
         log("mixed leg: warmup")
         if W > 0:
             m.run(1, W)
         m.synchronize()
+        log("mixed leg: autotune")
+        tuning_m = m.autotune() if not args.no_tune else None
         log("mixed leg: timed iterations")
         t0 = time.perf_counter()
         rn_m, _ = m.run(W + 1, K)

@@ -201,7 +201,7 @@ int mfcfd_run_graph(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, d
  * bound cloud(s) and the current state, every compiled variant of the
  * gather kernels -- block size 64/128/256 x minimum waves per SIMD (the
  * register cap: 2/4/6 for the q-derivative kernel, 1..4 for the fused flux
- * kernels), then for the chosen shape an LDS reservation that caps the
+ * kernels, 1..6 for their mixed-precision FP32 variants), then for the chosen shape an LDS reservation that caps the
  * resident waves per CU (fewer points in flight per XCD keep their
  * neighbours in that XCD's L2) -- and, in implicit mode, the sync-free
  * sweeps on 1, 1/2 or 1/4 of the resident grid; keeps the fastest.  Results do not depend on the

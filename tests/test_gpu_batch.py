@@ -106,3 +106,30 @@ def test_autotuned_shapes_leave_results_unchanged():
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
 
+
+
+def test_mixed_flux_variants_agree():
+    """Every compiled variant of the mixed form's fused flux kernels (block
+    64/128/256 x register caps of 1..6 waves per SIMD; the 5- and 6-wave caps
+    spill a few registers) does the same FP32 arithmetic per point, so runs
+    under each shape agree bit for bit (explicit, 20k shell cloud)."""
+    note("[gpu] mixed flux kernel variants: every shape vs the default")
+    cl = pc.ellipsoid_cloud(1600, 800, 17600, seed=21, order="morton")
+    mach, aoa, cfl = 0.84, 3.06, 0.5
+    p0 = prim0(cl, mach, aoa)
+    ref = None
+    for block, waves in [(0, 0), (64, 1), (128, 3), (256, 4), (128, 5), (256, 6), (64, 6)]:
+        s = solver.Solver(cl, mach, aoa, cfl, mode="explicit", precision="mixed")
+        s.upload("prim", p0)
+        s.set_tuning({"flux_interior_block": block, "flux_interior_waves": waves,
+                      "flux_wall_block": block, "flux_wall_waves": waves,
+                      "qderiv_block": 0, "qderiv_waves": 0, "sweep_div": 0})
+        rn, _ = s.run(1, 8)
+        got = (rn, s.download("prim"))
+        s.close()
+        if ref is None:
+            ref = got
+            continue
+        note(f"      block {block} waves {waves}")
+        assert np.array_equal(got[0], ref[0])
+        assert np.array_equal(got[1], ref[1])

@@ -30,35 +30,63 @@ __device__ __forceinline__ int xcd_block() {
 // index of this thread when the launch covers count items, blocks renumbered per XCD
 __device__ __forceinline__ int xcd_index() { return xcd_block() * (int)blockDim.x + (int)threadIdx.x; }
 
+// The value of v in the lane whose index differs in bit O.  Within aligned
+// groups of four lanes (O = 1, 2) this is a DPP quad permutation, one VALU
+// move per 32-bit word; wider partners go through the LDS crossbar
+// (__shfl_xor).  Every lane of each aligned group of four must be active.
+template <int O>
+__device__ __forceinline__ int lane_xor_b32(int v) {
+    static_assert(O == 1 || O == 2, "quad permutations only");
+    // quad_perm [1,0,3,2] and [2,3,0,1]
+    return __builtin_amdgcn_update_dpp(v, v, O == 1 ? 0xB1 : 0x4E, 0xf, 0xf, false);
+}
+template <int O, int G, typename V>
+__device__ __forceinline__ V xor_partner(V v) {
+    if constexpr (O > 2) {
+        return __shfl_xor(v, O, G);
+    } else if constexpr (sizeof(V) == 8) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = lane_xor_b32<O>((int)b), hi = lane_xor_b32<O>((int)(b >> 32));
+        return __builtin_bit_cast(V, ((long long)hi << 32) | (long long)(unsigned)lo);
+    } else {
+        return __builtin_bit_cast(V, lane_xor_b32<O>(__builtin_bit_cast(int, v)));
+    }
+}
+
 // Sum of v over the G consecutive lanes of an aligned lane group (G a power
 // of two).  Every lane of the group forms the same pairwise tree, so every
 // lane receives the same, order-determined value.
-template <int G, typename V>
+template <int G, int O = 1, typename V>
 __device__ __forceinline__ V group_sum(V v) {
-#pragma unroll
-    for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, G);
-    return v;
+    if constexpr (O < G) {
+        v += xor_partner<O, G>(v);
+        return group_sum<G, 2 * O>(v);
+    } else {
+        return v;
+    }
 }
 
 // group maximum / minimum with the comparisons of a sequential scan
 // (a NaN operand never replaces a value)
-template <int G, typename V>
+template <int G, int O = 1, typename V>
 __device__ __forceinline__ V group_max(V v) {
-#pragma unroll
-    for (int o = 1; o < G; o <<= 1) {
-        const V w = __shfl_xor(v, o, G);
+    if constexpr (O < G) {
+        const V w = xor_partner<O, G>(v);
         if (w > v) v = w;
+        return group_max<G, 2 * O>(v);
+    } else {
+        return v;
     }
-    return v;
 }
-template <int G, typename V>
+template <int G, int O = 1, typename V>
 __device__ __forceinline__ V group_min(V v) {
-#pragma unroll
-    for (int o = 1; o < G; o <<= 1) {
-        const V w = __shfl_xor(v, o, G);
+    if constexpr (O < G) {
+        const V w = xor_partner<O, G>(v);
         if (w < v) v = w;
+        return group_min<G, 2 * O>(v);
+    } else {
+        return v;
     }
-    return v;
 }
 
 template <typename T>

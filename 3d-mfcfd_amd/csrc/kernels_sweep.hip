@@ -100,10 +100,35 @@ __device__ __forceinline__ void wait_read(const double* v5, double out[5], int* 
     }
 }
 
+// one DPP move of a double (both 32-bit halves through v_mov_b32_dpp): a
+// lane whose source lane lies outside its row, or whose row is masked off,
+// reads 0
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wave (all active), in a fixed order, returned
+// to every lane.  Prefix sums within each row of 16 lanes (row_shr 1, 2, 4,
+// 8: lane 15 of a row holds the row's sum), row 0's sum into row 1 and row
+// 2's into row 3 (row_bcast:15), then lane 31's into rows 2 and 3
+// (row_bcast:31): lane 63 ends with the total, read back with readlane.
+// Each step is one VALU move per half, where a shuffle is a round trip
+// through the LDS crossbar.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, 64);
-    return v;
+    v = v + dpp_move<0x111, 0xf>(v);
+    v = v + dpp_move<0x112, 0xf>(v);
+    v = v + dpp_move<0x114, 0xf>(v);
+    v = v + dpp_move<0x118, 0xf>(v);
+    v = v + dpp_move<0x142, 0xa>(v);
+    v = v + dpp_move<0x143, 0xc>(v);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 template <bool FORWARD, bool UW, typename T>

@@ -20,8 +20,8 @@ and stores inputs and outputs as .npz:
 The reference's implicit sweep visits point_with_alias(N) = 0 and writes
 delUp(:,0)/delUn(:,0) just below those arrays.  Above glibc's mmap threshold
 (every production-size cloud) the writes land in page slack; below it they
-would overwrite the tail of the previously allocated array.  The driver
-therefore runs with glibc.malloc.mmap_threshold=4096 so small clouds are laid
+would overwrite the tail of the previously allocated array.  The fixture
+program (ref_driver) therefore runs with glibc.malloc.mmap_threshold=4096 so small clouds are laid
 out like large ones.
 
 --c1 additionally runs the 50k-point explicit configuration for 1000

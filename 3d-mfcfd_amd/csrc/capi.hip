@@ -130,7 +130,7 @@ struct mfcfd_ctx {
     int n = 0;
     // geometry, device order
     DevBuf<double4> pos;
-    DevBuf<double> frame, coef;
+    DevBuf<double> frame, coef, dgeo;   // dgeo: per direction list, D_inv's Cramer numerator and determinant
     DevBuf<int> status, nbhs, conn, dir_cnt, lst, alias, respos, perm, nwalk;
     DevBuf<long long> dir_off;
     DevBuf<unsigned> walk;
@@ -199,6 +199,7 @@ struct mfcfd_ctx {
         g.walk = walk.p;
         g.nwalk = nwalk.p;
         g.coef = coef.p;
+        g.dgeo = dgeo.p;
         g.alias = alias.p;
         g.respos = respos.p;
         g.perm = perm.p;
@@ -225,7 +226,7 @@ struct mfcfd_ctx {
         return s;
     }
     void release_all() {
-        DevBuf<double>* dbl[] = {&frame, &coef, &prim, &U, &q, &qm, &dq, &delt, &tsden, &flux_res, &delUp, &delUn,
+        DevBuf<double>* dbl[] = {&frame, &coef, &dgeo, &prim, &U, &q, &qm, &dq, &delt, &tsden, &flux_res, &delUp, &delUn,
                                  &dinv, &temp, &staging, &partial, &hist};
         for (auto* b : dbl) b->release();
         DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &nwalk,
@@ -796,6 +797,8 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
 
     if (c->coef.alloc(18ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "stencil coefficients");
     launch_stencil_coefficients(c->geom(), c->coef.p, c->fp.power, st);
+    if (c->dgeo.alloc(12ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "D_inv geometry");
+    launch_dinv_geometry(c->geom(), c->dgeo.p, c->fp.power, st);
     if ((rc = launch_error("stencil coefficients"))) return rc;
 
     // flow state (zero until uploaded; initial_conditions.f90:23-24 zero delUp/delUn)

@@ -60,6 +60,7 @@ struct DevGeom {
     const unsigned* walk;        // [kWalkCols n] entries the fused flux kernel visits, packed
     const int* nwalk;            // [n] used entries of walk
     const double* coef;          // [18n]
+    const double* dgeo;          // [12n] per direction list: D_inv's Cramer numerator and determinant
     const int* alias;
     const int* respos;           // position in the residual order (wall, then interior)
     const int* perm;             // device point -> host point (0-based)

@@ -88,6 +88,8 @@ void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st);
 // implicit_forward_and_reverse_sweeps.f90
 void launch_dinv(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                  hipStream_t st);
+// setup: the geometric part of every direction's D_inv term (12 doubles per point)
+void launch_dinv_geometry(const DevGeom& g, double* dgeo, double power, hipStream_t st);
 void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams& p,
                           const SweepSchedule& sch, hipStream_t st);
 // forward and backward sweeps without level barriers, then delUp = delUn

@@ -456,9 +456,15 @@ __global__ void __launch_bounds__(kBlock) k_explicit_sweep(DevGeom g, DevState s
 // implicit LU-SGS: implicit_D_inv_{interior,wall}.f90 and the alias-ordered
 // sweeps of implicit_forward_and_reverse_sweeps.f90:26-88
 // ---------------------------------------------------------------------------
+// The geometric part of one direction's D_inv term
+// (implicit_D_inv_interior.f90): over the direction list, the normal matrix of
+// the weighted least-squares fit and the right-hand side (sum dels*w,
+// sum delt*w, sum deln*w); returns the Cramer numerator of the list's axis
+// and the determinant.  Positions, frames and weights only, so it is computed
+// once, at bind.
 template <bool UW>
-__device__ double dinv_dir(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
-                           const Stencil& st, const Frame<double>& f, double as) {
+__device__ void dinv_geometry(const DevGeom& g, double power, int i, const Stencil& st, const Frame<double>& f,
+                              double& temp, double& det) {
     Geo6<double> geo = {0, 0, 0, 0, 0, 0};
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const double4 pi4 = g.pos[i];
@@ -470,7 +476,7 @@ __device__ double dinv_dir(const DevGeom& g, const DevState& s, const FlowParams
         const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
         const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
         const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
-        const double weights = wls_weight<UW>(dels, delt, deln, fp.power);
+        const double weights = wls_weight<UW>(dels, delt, deln, power);
         const double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
         geo.xx = geo.xx + dels * dels_w;
         geo.yy = geo.yy + delt * delt_w;
@@ -482,12 +488,34 @@ __device__ double dinv_dir(const DevGeom& g, const DevState& s, const FlowParams
         sy = sy + delt_w;
         sz = sz + deln_w;
     }
-    const double det = cramer_det(geo);
-    const double temp = cramer_num(geo, st.axis, sx, sy, sz);
-    // global-frame velocity component with the sound speed
-    // (implicit_D_inv_interior.f90:107-108,168-169)
-    const double vel = s.prim[5 * (long long)i + 1 + st.axis];
-    if (st.sign > 0) {
+    det = cramer_det(geo);
+    temp = cramer_num(geo, st.axis, sx, sy, sz);
+}
+
+// setup: the numerator and determinant of all six direction lists of every
+// point, dgeo[12 i + 2 d] and dgeo[12 i + 2 d + 1] (a list's geometry does
+// not depend on the point's class)
+template <bool UW>
+__global__ void __launch_bounds__(kBlock) k_dinv_geometry(DevGeom g, double* dgeo, double power) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const Frame<double> f = load_frame(g.frame, i);
+    for (int d = XPOS; d <= ZNEG; ++d) {
+        double temp, det;
+        dinv_geometry<UW>(g, power, i, dir_stencil(g, 0, d, i), f, temp, det);
+        dgeo[12 * (long long)i + 2 * d] = temp;
+        dgeo[12 * (long long)i + 2 * d + 1] = det;
+    }
+}
+
+// One direction's D_inv term from its precomputed geometry: the split
+// spectral radius of the global-frame velocity component with the sound
+// speed (implicit_D_inv_interior.f90:107-108,168-169) times numerator /
+// determinant, in the reference's operation order
+__device__ __forceinline__ double dinv_dir(const DevState& s, int i, int d, const double* gd, double as) {
+    const double temp = gd[2 * d], det = gd[2 * d + 1];
+    const double vel = s.prim[5 * (long long)i + 1 + (d >> 1)];
+    if ((d & 1) == 0) {
         const double temp1 = vel + as;
         return 0.50 * fabs(temp1 + fabs(temp1)) * temp / det;
     }
@@ -495,24 +523,22 @@ __device__ double dinv_dir(const DevGeom& g, const DevState& s, const FlowParams
     return -(0.50 * fabs(temp1 - fabs(temp1)) * temp / det);
 }
 
-template <bool UW>
-__global__ void __launch_bounds__(kBlock) k_dinv(DevGeom g, DevState s, FlowParams fp, const int* idx,
-                                                 int count) {
+__global__ void __launch_bounds__(kBlock) k_dinv(DevGeom g, DevState s, const int* idx, int count) {
     const int m = xcd_index();
     if (m >= count) return;
     const int i = idx[m];
     const int cls = g.status[i];
-    const Frame<double> f = load_frame(g.frame, i);
+    const double* gd = g.dgeo + 12 * (long long)i;
     const double delta_t = s.delt[i];
     const double as = __builtin_sqrt(1.40 * s.prim[5 * (long long)i + 4] / s.prim[5 * (long long)i]);
     double d[6];
     double temp;
     if (cls == 1) {
-        for (int dd = XPOS; dd <= YNEG; ++dd) d[dd] = dinv_dir<UW>(g, s, fp, i, dir_stencil(g, 1, dd, i), f, as);
-        d[4] = dinv_dir<UW>(g, s, fp, i, dir_stencil(g, 1, ZNEG, i), f, as);
+        for (int dd = XPOS; dd <= YNEG; ++dd) d[dd] = dinv_dir(s, i, dd, gd, as);
+        d[4] = dinv_dir(s, i, ZNEG, gd, as);
         temp = (1.0 / delta_t) - 2.0 * (d[0] + d[1] + d[2] + d[3] + d[4]);
     } else {
-        for (int dd = XPOS; dd <= ZNEG; ++dd) d[dd] = dinv_dir<UW>(g, s, fp, i, dir_stencil(g, 0, dd, i), f, as);
+        for (int dd = XPOS; dd <= ZNEG; ++dd) d[dd] = dinv_dir(s, i, dd, gd, as);
         temp = (1.0 / delta_t) - (d[0] + d[1] + d[2] + d[3] + d[4] + d[5]);
     }
     s.dinv[i] = 1.0 / temp;
@@ -1013,15 +1039,18 @@ void launch_dinv(const DevGeom& g, const DevState& s, const FlowParams& p, const
                  hipStream_t st) {
     const int* lists[2] = {tl.wall, tl.interior};
     const int counts[2] = {tl.n_wall, tl.n_interior};
+    (void)p;
     for (int c = 0; c < 2; ++c) {
         if (counts[c] <= 0) continue;
-        if (p.power == 0.0)
-            hipLaunchKernelGGL(k_dinv<true>, dim3(grid_for(counts[c], kBlock)), dim3(kBlock), 0, st, g, s, p,
-                               lists[c], counts[c]);
-        else
-            hipLaunchKernelGGL(k_dinv<false>, dim3(grid_for(counts[c], kBlock)), dim3(kBlock), 0, st, g, s, p,
-                               lists[c], counts[c]);
+        hipLaunchKernelGGL(k_dinv, dim3(grid_for(counts[c], kBlock)), dim3(kBlock), 0, st, g, s, lists[c], counts[c]);
     }
+}
+void launch_dinv_geometry(const DevGeom& g, double* dgeo, double power, hipStream_t st) {
+    if (g.n <= 0) return;
+    if (power == 0.0)
+        hipLaunchKernelGGL(k_dinv_geometry<true>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, dgeo, power);
+    else
+        hipLaunchKernelGGL(k_dinv_geometry<false>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, dgeo, power);
 }
 void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepSchedule& sch,
                           hipStream_t st) {

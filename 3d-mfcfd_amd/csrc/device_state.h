@@ -37,7 +37,7 @@ namespace mfcfd {
 constexpr int kConnCols = 27;
 // entries per point of the fused flux kernel's walk: conn, then the list
 // entries that are not an in-order subsequence of conn (22-column spill)
-constexpr int kWalkCols = 32;
+constexpr int kWalkCols = 64;
 // a walk entry packs the neighbour's device number above kWalkMaskBits bits
 // of direction-list membership (bit d: the entry is in list d)
 constexpr int kWalkMaskBits = 6;

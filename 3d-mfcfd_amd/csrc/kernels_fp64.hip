@@ -7,7 +7,7 @@
 // follows the reference routine named at it with the same association of
 // each product; the file is compiled with -ffp-contract=off.  The fused flux
 // kernels (kernels_flux_fused.hip) give the flux residual of every point
-// whose walk fits in 32 entries; this file keeps the list-by-list form for
+// whose walk fits in 64 entries; this file keeps the list-by-list form for
 // the rest and for the reference-order mode.
 #include <hip/hip_runtime.h>
 

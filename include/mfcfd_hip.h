@@ -245,7 +245,7 @@ int mfcfd_sweep_info(mfcfd_ctx *ctx, int *levels_fwd, int *levels_bwd, int *n_sw
  * (interior points whose lists partition conn), its general class (wall and
  * outer points, and interior points whose lists the 22-column spill
  * altered) and the list-by-list kernel (points whose conn and out-of-order
- * list entries together exceed 32) */
+ * list entries together exceed 64) */
 int mfcfd_point_classes(mfcfd_ctx *ctx, int *n_fused_interior, int *n_fused_wall, int *n_listwise);
 
 /* diagnostics: the same classes per point, host order (N ints):

@@ -37,7 +37,7 @@
 // conn, so its ZPOS list is skipped), each packed as k << 3 | direction.
 // Lane e takes entry e (e + 64, ... for longer lists), and the sum over all
 // entries (the sweep term is the sum of the directions' dG, times 2 at wall
-// points) is a fixed-order butterfly within the wave.  Each entry's
+// points) is a fixed-order DPP reduction within the wave (wave_sum).  Each entry's
 // contribution is (Gnew - Gold) * w (c0*dels + c1*delt + c2*deln) with the
 // Cramer cofactors c of the list (k_stencil_coef), which equals the
 // reference's Cramer solve on the summed products up to rounding.  T is the

@@ -164,6 +164,9 @@ struct mfcfd_ctx {
     int n_inst = 1;
     DevBuf<int> inst_id, inst_beg, inst_end, hist_ctr;
     DevBuf<double> inst_fs, inst_norm, ressq, inst_hist;
+    DevBuf<int> chunk_beg, chunk_end, chunk_ptr;   // per-cloud residual chunks
+    DevBuf<double> chunk_sum;
+    int n_chunks = 0;
     std::vector<int> inst_count;
     // launch shapes chosen by mfcfd_autotune (defaults until then)
     LaunchShapes shapes;
@@ -239,9 +242,9 @@ struct mfcfd_ctx {
         qm32.release();
         dq32.release();
         sweep_ctr.release();
-        DevBuf<int>* ib[] = {&inst_id, &inst_beg, &inst_end, &hist_ctr};
+        DevBuf<int>* ib[] = {&inst_id, &inst_beg, &inst_end, &hist_ctr, &chunk_beg, &chunk_end, &chunk_ptr};
         for (auto* b : ib) b->release();
-        DevBuf<double>* db[] = {&inst_fs, &inst_norm, &ressq, &inst_hist};
+        DevBuf<double>* db[] = {&inst_fs, &inst_norm, &ressq, &inst_hist, &chunk_sum};
         for (auto* b : db) b->release();
         dir_off.release();
         walk.release();
@@ -947,6 +950,24 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
         if ((rc = upload(c->inst_beg, beg.data(), beg.size(), st))) return rc;
         if ((rc = upload(c->inst_end, end.data(), end.size(), st))) return rc;
         if ((rc = upload(c->inst_norm, norm.data(), norm.size(), st))) return rc;
+        // residual chunks: each cloud's range cut into kResidualChunk points
+        std::vector<int> cb, ce, cp(c->n_inst + 1, 0);
+        for (int b = 0; b < c->n_inst; ++b) {
+            for (int a0 = beg[b]; a0 < end[b]; a0 += kResidualChunk) {
+                cb.push_back(a0);
+                ce.push_back(std::min(end[b], a0 + kResidualChunk));
+            }
+            cp[b + 1] = (int)cb.size();
+        }
+        c->n_chunks = (int)cb.size();
+        if (cb.empty()) {   // keep the buffers allocated
+            cb.push_back(0);
+            ce.push_back(0);
+        }
+        if ((rc = upload(c->chunk_beg, cb.data(), cb.size(), st))) return rc;
+        if ((rc = upload(c->chunk_end, ce.data(), ce.size(), st))) return rc;
+        if ((rc = upload(c->chunk_ptr, cp.data(), cp.size(), st))) return rc;
+        if (c->chunk_sum.alloc(cb.size()) != hipSuccess) return fail(MFCFD_ENOMEM, "residual chunk sums");
         if (c->batch) {
             if ((rc = upload(c->inst_id, id.data(), id.size(), st))) return rc;
             if ((rc = upload(c->inst_fs, fs, 2ULL * n_inst, st))) return rc;
@@ -1240,6 +1261,11 @@ int mfcfd_run_graph(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, dou
     ir.n_inst = c->n_inst;
     ir.inst_hist = c->inst_hist.p;
     ir.counter = c->hist_ctr.p;
+    ir.chunk_beg = c->chunk_beg.p;
+    ir.chunk_end = c->chunk_end.p;
+    ir.chunk_ptr = c->chunk_ptr.p;
+    ir.n_chunks = c->n_chunks;
+    ir.chunk_sum = c->chunk_sum.p;
     if (c->graph_stale || !c->graph_exec) {
         if (c->graph_exec) {
             (void)hipGraphExecDestroy(c->graph_exec);

@@ -100,6 +100,10 @@ int sweep_async_grid(int count, bool unit_weight, bool mixed);
 // state_update.f90 (+ residual reduction into *rec; hist may be null)
 // per-cloud residuals of a batch, recorded in row *counter of inst_hist
 // (n_inst per row); the counter advances once per state update
+// Each cloud's range is cut into chunks of at most kResidualChunk points;
+// chunk c covers [chunk_beg[c], chunk_end[c]), and cloud b owns chunks
+// chunk_ptr[b] .. chunk_ptr[b+1]
+constexpr int kResidualChunk = 2048;
 struct InstanceResidual {
     double* ressq;       // [n] per-point squared density change
     const int* beg;      // [n_inst] device-order point range of each cloud
@@ -108,6 +112,11 @@ struct InstanceResidual {
     int n_inst;
     double* inst_hist;
     int* counter;
+    const int* chunk_beg;
+    const int* chunk_end;
+    const int* chunk_ptr;   // [n_inst + 1]
+    int n_chunks;
+    double* chunk_sum;      // [n_chunks] scratch
 };
 // hist[hist_index] (or hist[*ir->counter] when ir is given) receives res_new;
 // with_explicit_sweep: do the explicit form's sweep (launch_explicit_sweep)

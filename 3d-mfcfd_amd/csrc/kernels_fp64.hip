@@ -871,13 +871,33 @@ __global__ void __launch_bounds__(1024) k_residual_final(const double* partial, 
 // residual of each cloud of a batch (state_update.f90:39-44,115 over the
 // cloud's own points): block b sums its cloud's per-point terms in a fixed
 // order; the history row is the iteration counter's
-__global__ void __launch_bounds__(kBlock) k_instance_residual(const double* ressq, const int* beg, const int* end,
+// Per-cloud residuals of a batch in two fixed-order passes: every chunk of a
+// cloud's range (in order per thread, then a pairwise tree), then every
+// cloud over its chunks' sums the same way.  The chunks spread each cloud
+// over many workgroups, where one workgroup per cloud left the largest
+// cloud's sum latency-bound.
+__global__ void __launch_bounds__(kBlock) k_chunk_sums(const double* ressq, const int* chunk_beg,
+                                                       const int* chunk_end, double* chunk_sum) {
+    __shared__ double ssum[kBlock];
+    const int c = blockIdx.x;
+    double acc = 0.0;
+    for (int k = chunk_beg[c] + threadIdx.x; k < chunk_end[c]; k += kBlock) acc = acc + ressq[k];
+    ssum[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) ssum[threadIdx.x] = ssum[threadIdx.x] + ssum[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) chunk_sum[c] = ssum[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_instance_residual(const double* chunk_sum, const int* chunk_ptr,
                                                              const double* norm, int n_inst, double* inst_hist,
                                                              const int* counter) {
     __shared__ double ssum[kBlock];
     const int b = blockIdx.x;
     double acc = 0.0;
-    for (int k = beg[b] + threadIdx.x; k < end[b]; k += kBlock) acc = acc + ressq[k];
+    for (int c = chunk_ptr[b] + threadIdx.x; c < chunk_ptr[b + 1]; c += kBlock) acc = acc + chunk_sum[c];
     ssum[threadIdx.x] = acc;
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {
@@ -1096,8 +1116,11 @@ void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& 
                        (const int*)partial_pos, (const int*)partial_pt, (const double*)partial_max, n_partial,
                        g.n, rec, hist, hist_index, ir ? (const int*)ir->counter : (const int*)nullptr);
     if (ir) {
-        hipLaunchKernelGGL(k_instance_residual, dim3(ir->n_inst), dim3(kBlock), 0, st, (const double*)ir->ressq,
-                           ir->beg, ir->end, ir->norm, ir->n_inst, ir->inst_hist, (const int*)ir->counter);
+        if (ir->n_chunks > 0)
+            hipLaunchKernelGGL(k_chunk_sums, dim3(ir->n_chunks), dim3(kBlock), 0, st, (const double*)ir->ressq,
+                               ir->chunk_beg, ir->chunk_end, ir->chunk_sum);
+        hipLaunchKernelGGL(k_instance_residual, dim3(ir->n_inst), dim3(kBlock), 0, st, (const double*)ir->chunk_sum,
+                           ir->chunk_ptr, ir->norm, ir->n_inst, ir->inst_hist, (const int*)ir->counter);
         hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, st, ir->counter);
     }
 }

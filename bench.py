@@ -97,6 +97,14 @@ def copy_probe(mib):
     log(f"copy probe: 3 launches of {mib} MiB read + {mib} MiB written")
 
 
+def class_counts(cls, status):
+    """Points per flux kernel class: fused interior, fused general (and how many
+    of those are interior points whose lists the spill altered), list-by-list."""
+    return {"fused_interior": int(np.sum(cls == 0)), "fused_general": int(np.sum(cls == 1)),
+            "fused_general_interior": int(np.sum((cls == 1) & (status == pc.STATUS_INTERIOR))),
+            "listwise": int(np.sum(cls == 2))}
+
+
 def bench_batch(args):
     """64 independent clouds (log-uniform 20k..500k points, aspect 2..6,
     Mach 0.3..0.85, AoA -5..10 deg) in one context, the iteration captured
@@ -176,6 +184,7 @@ def bench_batch(args):
         "tuning": f64["tuning"],
         "instance_residual_last": {"min": float(last.min()), "max": float(last.max()),
                                    "finite": int(np.isfinite(last).sum())},
+        "point_classes": class_counts(f64["cls"], status),
     }
     print(json.dumps(out), flush=True)
 
@@ -343,6 +352,7 @@ def main():
         "mixed": mixed,
         "tuning": tuning,
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
+        "point_classes": class_counts(cls, cloud.status),
         "step_algorithmic_gbs": round(step_gbs, 2),
         "step_hbm_frac": round(step_gbs / roofline.HBM_PEAK_GBS, 5),
     }

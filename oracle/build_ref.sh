@@ -8,23 +8,30 @@
 # The reference sizes every array with the compile-time PARAMETER max_points
 # and takes the flow conditions as PARAMETERs (parameter.f90:8-23), so each
 # configuration gets its own build: parameter.f90 is derived into the build
-# directory with these values substituted.  The canonical implicit build is
-# the alias-ordered sweep (implicit_forward_and_reverse_sweeps.f90) with
-# state_update.f90; the explicit build links oracle/ref_explicit_sweep.f90 in
-# place of the sweep module.  flang -O2 for the default x86-64 target, no FMA.
+# directory with these values substituted (and, when given, the WLS weight
+# exponent power and the q-derivative inner_iterations, parameter.f90:21,24).
+# The canonical implicit build is the alias-ordered sweep
+# (implicit_forward_and_reverse_sweeps.f90) with state_update.f90; the
+# alternative implicit build (implicit_backup) is the type-ordered sweep
+# that also sweeps outer points (implicit_forward_and_reverse_sweeps_backup.f90)
+# with state_update_implicit.f90; the explicit build links
+# oracle/ref_explicit_sweep.f90 in place of the sweep module.  flang -O2 for
+# the default x86-64 target, no FMA.
 #
-# usage: build_ref.sh <tag> <max_points> <max_iters> <mach> <aoa> <cfl> <explicit|implicit>
+# usage: build_ref.sh <tag> <max_points> <max_iters> <mach> <aoa> <cfl>
+#                     <explicit|implicit|implicit_backup> [<power> [<inner_iterations>]]
 set -euo pipefail
 REF=${MFCFD_REFERENCE_DIR:-/root/reference}
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 FC=${FC:-/opt/rocm/lib/llvm/bin/flang}
 FFLAGS=${FFLAGS:-"-O2 -ffp-contract=off"}
 
-if [ $# -ne 7 ]; then
-    echo "usage: $0 <tag> <max_points> <max_iters> <mach> <aoa> <cfl> <explicit|implicit>" >&2
+if [ $# -lt 7 ] || [ $# -gt 9 ]; then
+    echo "usage: $0 <tag> <max_points> <max_iters> <mach> <aoa> <cfl> <explicit|implicit|implicit_backup> [power [inner_iterations]]" >&2
     exit 2
 fi
 tag=$1; npts=$2; iters=$3; mach=$4; aoa=$5; cfl=$6; mode=$7
+power=${8:-0.0}; inner=${9:-0}
 [ -d "$REF" ] || { echo "reference sources not found at $REF" >&2; exit 3; }
 out="$HERE/_ref/$tag"
 mkdir -p "$out"
@@ -35,15 +42,23 @@ sed -e "s/^\(\s*integer, parameter :: max_points = \)26133/\1$npts/" \
     -e "s/^\(\s*real\*8, parameter :: Mach = \)2.0d0/\1$(dlit "$mach")/" \
     -e "s/^\(\s*real\*8, parameter :: aoa = \)0.0d0/\1$(dlit "$aoa")/" \
     -e "s/^\(\s*real\*8, parameter :: CFL = \)5.0d0/\1$(dlit "$cfl")/" \
+    -e "s/^\(\s*real\*8, parameter :: power = \)0.0d0/\1$(dlit "$power")/" \
+    -e "s/^\(\s*integer, parameter :: inner_iterations = \)0/\1$inner/" \
     "$REF/parameter.f90" > "$out/parameter.f90"
 grep -q "max_points = $npts" "$out/parameter.f90" || { echo "parameter substitution failed" >&2; exit 4; }
+grep -q "power = $(dlit "$power")" "$out/parameter.f90" || { echo "power substitution failed" >&2; exit 4; }
+grep -q "inner_iterations = $inner\$" "$out/parameter.f90" || { echo "inner_iterations substitution failed" >&2; exit 4; }
 
+update="$REF/state_update.f90"
 if [ "$mode" = explicit ]; then
     sweep="$HERE/ref_explicit_sweep.f90"
 elif [ "$mode" = implicit ]; then
     sweep="$REF/implicit_forward_and_reverse_sweeps.f90"
+elif [ "$mode" = implicit_backup ]; then
+    sweep="$REF/implicit_forward_and_reverse_sweeps_backup.f90"
+    update="$REF/state_update_implicit.f90"
 else
-    echo "mode must be explicit or implicit" >&2; exit 2
+    echo "mode must be explicit, implicit or implicit_backup" >&2; exit 2
 fi
 
 # dependency order of the canonical file set (module 'use' graph)
@@ -68,7 +83,7 @@ srcs=(
   "$REF/implicit_UdelU_backward_sweep_outer.f90"
   "$sweep"
   "$REF/primitive_to_conserved_vector.f90" "$REF/conserved_to_primitive_vector.f90"
-  "$REF/conserved_vector_Ubar.f90" "$REF/state_update.f90" "$REF/fpi_solver.f90"
+  "$REF/conserved_vector_Ubar.f90" "$update" "$REF/fpi_solver.f90"
   "$REF/generate_connectivity.f90" "$REF/implicit_aliasing.f90"
   "$REF/point_preprocessor.f90" "$REF/initial_conditions.f90"
   "$HERE/ref_driver.f90"

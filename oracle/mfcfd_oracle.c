@@ -15,10 +15,16 @@
  * -(a*b), which rounds identically to (-a)*b.
  *
  * Parity pins (tests/test_oracle_*.py): bit-for-bit equality with the
- * reference compiled by flang (oracle/_ref, built by oracle/Makefile) on the
- * golden clouds; closed forms (KFVS and octant flux identities, q <-> prim
- * and U <-> prim round trips, WLS exactness on linear fields, limiter special
- * cases, freestream preservation, brute-force split stencils and alias BFS).
+ * reference compiled by flang (oracle/_ref, built by oracle/build_ref.sh) on
+ * the golden clouds -- the canonical explicit and implicit forms, supersonic
+ * inlet/outlet points in both forms, power = 2 with one inner q-derivative
+ * iteration in both forms, and the alternative implicit pairing (outer
+ * points swept, state_update_implicit.f90) -- and on the first iterations of
+ * the 50k-point history; closed forms (KFVS and octant flux identities,
+ * q <-> prim and U <-> prim round trips, WLS exactness on linear fields,
+ * limiter special cases, freestream preservation, brute-force split stencils
+ * and alias BFS).  Every function below is reached by at least one of the
+ * bit-for-bit fixtures.
  */
 #include "mfcfd_oracle.h"
 
@@ -256,6 +262,22 @@ static double cramer_num(const geo6 *g, int col, double a, double b, double c)
          + g->zx * (g->xy * b - a * g->yy);
 }
 
+/* dist**power (q_derivatives.f90:63, interior_flux_dGxpos.f90:78, ...) as
+ * the flang build evaluates it.  parameter.f90:21 makes power a compile-time
+ * constant, and flang's LLVM reduces the constant exponents 0, 0.5, 1, 2 and
+ * 3 to 1, sqrt(d), d, d*d and (d*d)*d (x86 code of the reference objects
+ * built here); any other exponent calls libm's pow.  glibc's pow(d, 2) is
+ * not always d*d rounded once, so the reduction matters bit for bit. */
+static double ref_pow(double d, double power)
+{
+    if (power == 0.0) return 1.0;
+    if (power == 0.5) return sqrt(d);
+    if (power == 1.0) return d;
+    if (power == 2.0) return d * d;
+    if (power == 3.0) return d * d * d;
+    return pow(d, power);
+}
+
 void orc_eval_q_derivatives(orc_state *s)
 {
     /* q_derivatives.f90:17-110 */
@@ -277,7 +299,7 @@ void orc_eval_q_derivatives(orc_state *s)
             double dely = s->y[nbh] - y_i;
             double delz = s->z[nbh] - z_i;
             double dist = sqrt(delx * delx + dely * dely + delz * delz);
-            double weights = 1.0 / pow(dist, s->power);
+            double weights = 1.0 / ref_pow(dist, s->power);
             g.xx = g.xx + delx * delx * weights;
             g.yy = g.yy + dely * dely * weights;
             g.zz = g.zz + delz * delz * weights;
@@ -298,7 +320,7 @@ void orc_eval_q_derivatives(orc_state *s)
                 DQ(s, i, r, c) = cramer_num(&g, c, sx[r], sy[r], sz[r]) * one_by_det;
     }
 
-    /* q_derivatives.f90:116-201 (dead while inner_iterations = 0) */
+    /* q_derivatives.f90:116-201 (inner_iterations > 0 only) */
     for (int it = 0; it < s->inner_iterations; ++it) {
         for (int i = 0; i < s->n; ++i) {
             double x_i = s->x[i], y_i = s->y[i], z_i = s->z[i];
@@ -310,7 +332,7 @@ void orc_eval_q_derivatives(orc_state *s)
                 double dely = s->y[nbh] - y_i;
                 double delz = s->z[nbh] - z_i;
                 double dist = sqrt(delx * delx + dely * dely + delz * delz);
-                double weights = 1.0 / pow(dist, s->power);
+                double weights = 1.0 / ref_pow(dist, s->power);
                 g.xx = g.xx + delx * delx * weights;
                 g.yy = g.yy + dely * dely * weights;
                 g.zz = g.zz + delz * delz * weights;
@@ -544,7 +566,7 @@ static void stencil_dG(const orc_state *s, int i, const stencil_t *st, double G[
         double delt = delx * t2[0] + dely * t2[1] + delz * t2[2];
         double deln = delx * n[0] + dely * n[1] + delz * n[2];
         double dist = sqrt(dels * dels + delt * delt + deln * deln);
-        double weights = 1.0 / pow(dist, s->power);
+        double weights = 1.0 / ref_pow(dist, s->power);
         double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
         g.xx = g.xx + dels * dels_w;
         g.yy = g.yy + delt * delt_w;
@@ -670,7 +692,7 @@ static double dinv_dir(const orc_state *s, int i, const stencil_t *st, double as
         double delt = delx * t2[0] + dely * t2[1] + delz * t2[2];
         double deln = delx * n[0] + dely * n[1] + delz * n[2];
         double dist = sqrt(dels * dels + delt * delt + deln * deln);
-        double weights = 1.0 / pow(dist, s->power);
+        double weights = 1.0 / ref_pow(dist, s->power);
         double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
         g.xx = g.xx + dels * dels_w;
         g.yy = g.yy + delt * delt_w;
@@ -722,6 +744,34 @@ double orc_implicit_D_inv(const orc_state *s, int i)
     return 1.0 / temp;
 }
 
+/* implicit_D_inv_outer.f90:17-363, reached only from the alternative sweep
+ * driver (implicit_forward_and_reverse_sweeps_backup.f90).  x/y over the
+ * split lists, z over the full conn; the negative halves enter with a plus
+ * sign (:171, :295) and the sum is (1/delt) - dGxp + dGxn - dGyp + dGyn - dGzp
+ * (:360).  The routine's local `delt` is both the time step (:46) and the
+ * loop's tan2 projection (:73, ...), so the 1/delt of :360 is the projection
+ * of the last conn neighbour's offset (the time step when nbhs = 0). */
+static double implicit_D_inv_outer(const orc_state *s, int i)
+{
+    double delt = s->delt[i];
+    double as = sqrt(1.40 * P5(s->prim, i, 4) / P5(s->prim, i, 0));
+    stencil_t st;
+    st = split_stencil(s, i, 0, +1, FK_OUTER); double dGxp = dinv_dir(s, i, &st, as);
+    st = split_stencil(s, i, 0, -1, FK_OUTER); double dGxn = -dinv_dir(s, i, &st, as);
+    st = split_stencil(s, i, 1, +1, FK_OUTER); double dGyp = dinv_dir(s, i, &st, as);
+    st = split_stencil(s, i, 1, -1, FK_OUTER); double dGyn = -dinv_dir(s, i, &st, as);
+    st = full_conn_stencil(s, i, 2, +1, FK_HALF); double dGzp = dinv_dir(s, i, &st, as);
+    if (s->nbhs[i] > 0) {
+        int k = CONN(s, i, s->nbhs[i] - 1) - 1;
+        double delx = s->x[k] - s->x[i];
+        double dely = s->y[k] - s->y[i];
+        double delz = s->z[k] - s->z[i];
+        delt = delx * F3(s->tan2, i, 0) + dely * F3(s->tan2, i, 1) + delz * F3(s->tan2, i, 2);
+    }
+    double temp = (1.0 / delt) - dGxp + dGxn - dGyp + dGyn - dGzp;
+    return 1.0 / temp;
+}
+
 /* get_primitive_vector.f90:10-41: local-frame U of point k -> global prim */
 static void get_primitive_vector(const orc_state *s, int k, double U[5], double prim[5])
 {
@@ -761,7 +811,7 @@ static void sweep_dir(const orc_state *s, int i, const stencil_t *st, int forwar
         double delt = delx * t2[0] + dely * t2[1] + delz * t2[2];
         double deln = delx * n[0] + dely * n[1] + delz * n[2];
         double dist = sqrt(dels * dels + delt * delt + deln * deln);
-        double weights = 1.0 / pow(dist, s->power);
+        double weights = 1.0 / ref_pow(dist, s->power);
         double dels_w = dels * weights, delt_w = delt * weights, deln_w = deln * weights;
         g.xx = g.xx + dels * dels_w;
         g.yy = g.yy + delt * delt_w;
@@ -818,8 +868,26 @@ static void sweep_term(const orc_state *s, int i, int forward, double out[5])
         out[r] = d[0][r] + d[1][r] + d[2][r] + d[3][r] + d[4][r] + d[5][r];
 }
 
-/* nearest neighbour of a supersonic point (sweeps.f90:39-50) */
-static int nearest_neighbour(const orc_state *s, int p, double start)
+/* implicit_LdelU_forward_sweep_outer.f90:20-406 and
+ * implicit_UdelU_backward_sweep_outer.f90:19-406 (alternative driver only):
+ * x/y octant fluxes Go over the split lists, z flux_Gzp over the full conn,
+ * each dG = temp/det, summed without a factor */
+static void sweep_term_outer(const orc_state *s, int i, int forward, double out[5])
+{
+    double d[5][5];
+    stencil_t st;
+    st = split_stencil(s, i, 0, +1, FK_OUTER); sweep_dir(s, i, &st, forward, d[0]);
+    st = split_stencil(s, i, 0, -1, FK_OUTER); sweep_dir(s, i, &st, forward, d[1]);
+    st = split_stencil(s, i, 1, +1, FK_OUTER); sweep_dir(s, i, &st, forward, d[2]);
+    st = split_stencil(s, i, 1, -1, FK_OUTER); sweep_dir(s, i, &st, forward, d[3]);
+    st = full_conn_stencil(s, i, 2, +1, FK_HALF); sweep_dir(s, i, &st, forward, d[4]);
+    for (int r = 0; r < 5; ++r) out[r] = d[0][r] + d[1][r] + d[2][r] + d[3][r] + d[4][r];
+}
+
+/* nearest neighbour of a supersonic point (sweeps.f90:39-50); skip_wall
+ * adds the status /= 1 filter of the alternative driver and
+ * state_update_implicit.f90:72,89 */
+static int nearest_neighbour_f(const orc_state *s, int p, double start, int skip_wall)
 {
     double min_dist = start;
     int q = p;
@@ -829,12 +897,17 @@ static int nearest_neighbour(const orc_state *s, int p, double start)
         double dy = s->y[nbh] - s->y[p];
         double dz = s->z[nbh] - s->z[p];
         double ds = sqrt(dx * dx + dy * dy + dz * dz);
-        if (ds < min_dist) {
+        if (ds < min_dist && (!skip_wall || s->status[nbh] != 1)) {
             min_dist = ds;
             q = nbh;
         }
     }
     return q;
+}
+
+static int nearest_neighbour(const orc_state *s, int p, double start)
+{
+    return nearest_neighbour_f(s, p, start, 0);
 }
 
 void orc_implicit_sweeps(orc_state *s)
@@ -877,6 +950,80 @@ void orc_implicit_sweeps(orc_state *s)
     /* :86-88 */
     for (int i = 0; i < n; ++i)
         for (int r = 0; r < 5; ++r) P5(s->delUp, i, r) = P5(s->delUn, i, r);
+}
+
+/* implicit_forward_and_reverse_sweeps_backup.f90:16-141: the forward sweep
+ * visits wall, interior, then outer points, each class in index order, then
+ * the supersonic inlet and outlet points (nearest non-wall neighbour's
+ * delUp); the backward sweep visits the same classes in reverse order.  The
+ * alias comparisons inside LdelU/UdelU are unchanged, so a neighbour with a
+ * smaller alias that this order visits later contributes its delUp from
+ * before the sweep.  No delUp = delUn copy: state_update_implicit.f90:98-100
+ * makes it. */
+void orc_implicit_sweeps_backup(orc_state *s)
+{
+    double D_inv, T[5];
+    for (int m = 0; m < s->wall_points; ++m) {
+        int k = s->wall_points_index[m] - 1;
+        D_inv = orc_implicit_D_inv(s, k);
+        sweep_term(s, k, 1, T);
+        for (int r = 0; r < 5; ++r) P5(s->delUp, k, r) = -(D_inv * (P5(s->flux_res, k, r) + T[r]));
+    }
+    for (int m = 0; m < s->interior_points; ++m) {
+        int k = s->interior_points_index[m] - 1;
+        D_inv = orc_implicit_D_inv(s, k);
+        sweep_term(s, k, 1, T);
+        for (int r = 0; r < 5; ++r) P5(s->delUp, k, r) = -(D_inv * (P5(s->flux_res, k, r) + T[r]));
+    }
+    for (int m = 0; m < s->outer_points; ++m) {
+        int k = s->outer_points_index[m] - 1;
+        D_inv = implicit_D_inv_outer(s, k);
+        sweep_term_outer(s, k, 1, T);
+        for (int r = 0; r < 5; ++r) P5(s->delUp, k, r) = -(D_inv * (P5(s->flux_res, k, r) + T[r]));
+    }
+    for (int m = 0; m < s->supersonic_inlet_points; ++m) {
+        int k = s->supersonic_inlet_points_index[m] - 1;
+        int p = nearest_neighbour_f(s, k, 100000.0, 1);
+        for (int r = 0; r < 5; ++r) P5(s->delUp, k, r) = P5(s->delUp, p, r);
+    }
+    for (int m = 0; m < s->supersonic_outlet_points; ++m) {
+        int k = s->supersonic_outlet_points_index[m] - 1;
+        int p = nearest_neighbour_f(s, k, 100000.0, 1);
+        for (int r = 0; r < 5; ++r) P5(s->delUp, k, r) = P5(s->delUp, p, r);
+    }
+    for (int m = s->wall_points - 1; m >= 0; --m) {
+        int k = s->wall_points_index[m] - 1;
+        D_inv = orc_implicit_D_inv(s, k);
+        sweep_term(s, k, 0, T);
+        for (int r = 0; r < 5; ++r) P5(s->delUn, k, r) = P5(s->delUp, k, r) - D_inv * T[r];
+    }
+    for (int m = s->interior_points - 1; m >= 0; --m) {
+        int k = s->interior_points_index[m] - 1;
+        D_inv = orc_implicit_D_inv(s, k);
+        sweep_term(s, k, 0, T);
+        for (int r = 0; r < 5; ++r) P5(s->delUn, k, r) = P5(s->delUp, k, r) - D_inv * T[r];
+    }
+    for (int m = s->outer_points - 1; m >= 0; --m) {
+        int k = s->outer_points_index[m] - 1;
+        D_inv = implicit_D_inv_outer(s, k);
+        sweep_term_outer(s, k, 0, T);
+        for (int r = 0; r < 5; ++r) P5(s->delUn, k, r) = P5(s->delUp, k, r) - D_inv * T[r];
+    }
+    for (int m = s->supersonic_inlet_points - 1; m >= 0; --m) {
+        int k = s->supersonic_inlet_points_index[m] - 1;
+        int p = nearest_neighbour_f(s, k, 100000.0, 1);
+        for (int r = 0; r < 5; ++r) P5(s->delUn, k, r) = P5(s->delUn, p, r);
+    }
+    for (int m = s->supersonic_outlet_points - 1; m >= 0; --m) {
+        int k = s->supersonic_outlet_points_index[m] - 1;
+        int p = nearest_neighbour_f(s, k, 100000.0, 1);
+        for (int r = 0; r < 5; ++r) P5(s->delUn, k, r) = P5(s->delUn, p, r);
+    }
+}
+
+double orc_implicit_D_inv_outer(const orc_state *s, int i)
+{
+    return implicit_D_inv_outer(s, i);
 }
 
 void orc_explicit_sweep(orc_state *s)
@@ -1004,6 +1151,63 @@ void orc_state_update(orc_state *s)
     s->res_new = sqrt(sum_res_sqr) / s->n;
 }
 
+/* state_update_implicit.f90:14-102, the update paired with the alternative
+ * sweep: wall and interior U + delUn without the wall's U(4) = 0 and without
+ * a per-point delUp copy, outer Ubar + delUn (:45-47), supersonic points
+ * copy the nearest non-wall neighbour's prim (:62-96), then delUp = delUn
+ * for every point (:98-100). */
+void orc_state_update_implicit(orc_state *s)
+{
+    double sum_res_sqr = 0.0;
+    s->max_res = 0.0;
+    for (int m = 0; m < s->wall_points; ++m) {
+        int k = s->wall_points_index[m] - 1;
+        double temp = P5(s->U, k, 0);
+        double U[5];
+        for (int r = 0; r < 5; ++r) U[r] = P5(s->U, k, r) + P5(s->delUn, k, r);
+        double res_sqr = (U[0] - temp) * (U[0] - temp);
+        if (res_sqr > s->max_res) {
+            s->max_res = res_sqr;
+            s->max_res_point = k + 1;
+        }
+        sum_res_sqr = sum_res_sqr + res_sqr;
+        conserved_to_primitive(s, k, U);
+    }
+    for (int m = 0; m < s->outer_points; ++m) {
+        int k = s->outer_points_index[m] - 1;
+        double U[5];
+        orc_conserved_vector_Ubar(s, k, U);
+        for (int r = 0; r < 5; ++r) U[r] = U[r] + P5(s->delUn, k, r);
+        conserved_to_primitive(s, k, U);
+    }
+    for (int m = 0; m < s->interior_points; ++m) {
+        int k = s->interior_points_index[m] - 1;
+        double temp = P5(s->U, k, 0);
+        double U[5];
+        for (int r = 0; r < 5; ++r) U[r] = P5(s->U, k, r) + P5(s->delUn, k, r);
+        double res_sqr = (U[0] - temp) * (U[0] - temp);
+        if (res_sqr > s->max_res) {
+            s->max_res = res_sqr;
+            s->max_res_point = k + 1;
+        }
+        sum_res_sqr = sum_res_sqr + res_sqr;
+        conserved_to_primitive(s, k, U);
+    }
+    for (int m = 0; m < s->supersonic_outlet_points; ++m) {
+        int k = s->supersonic_outlet_points_index[m] - 1;
+        int p = nearest_neighbour_f(s, k, 100000.0, 1);
+        for (int r = 0; r < 5; ++r) P5(s->prim, k, r) = P5(s->prim, p, r);
+    }
+    for (int m = 0; m < s->supersonic_inlet_points; ++m) {
+        int k = s->supersonic_inlet_points_index[m] - 1;
+        int p = nearest_neighbour_f(s, k, 100000.0, 1);
+        for (int r = 0; r < 5; ++r) P5(s->prim, k, r) = P5(s->prim, p, r);
+    }
+    s->res_new = sqrt(sum_res_sqr) / s->n;
+    for (int i = 0; i < s->n; ++i)
+        for (int r = 0; r < 5; ++r) P5(s->delUp, i, r) = P5(s->delUn, i, r);
+}
+
 /* ------------------------------------------------------------------------ */
 /* the iteration                                                            */
 /* ------------------------------------------------------------------------ */
@@ -1016,9 +1220,14 @@ void orc_fpi_solver(orc_state *s, int t, int implicit_mode)
     orc_eval_q_derivatives(s);
     orc_timestep_delt(s);
     orc_eval_flux_residual(s);
-    if (implicit_mode) orc_implicit_sweeps(s);
-    else orc_explicit_sweep(s);
-    orc_state_update(s);
+    if (implicit_mode == ORC_IMPLICIT_BACKUP) {
+        orc_implicit_sweeps_backup(s);
+        orc_state_update_implicit(s);
+    } else {
+        if (implicit_mode) orc_implicit_sweeps(s);
+        else orc_explicit_sweep(s);
+        orc_state_update(s);
+    }
     if (t <= 2) {
         s->res_old = s->res_new;
         s->residue = 0.0;

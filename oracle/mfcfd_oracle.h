@@ -75,6 +75,13 @@ void orc_eval_flux_residual(orc_state *s);       /* flux_residual.f90:30-84 */
 void orc_implicit_sweeps(orc_state *s);          /* implicit_forward_and_reverse_sweeps.f90:16-89 */
 void orc_explicit_sweep(orc_state *s);           /* explicit form: delUn = -delt*flux_res */
 void orc_state_update(orc_state *s);             /* state_update.f90:15-118 */
+/* alternative implicit pairing (implicit_forward_and_reverse_sweeps_backup.f90
+ * with state_update_implicit.f90) */
+void orc_implicit_sweeps_backup(orc_state *s);   /* ..._backup.f90:16-141 */
+void orc_state_update_implicit(orc_state *s);    /* state_update_implicit.f90:14-102 */
+/* implicit_mode: 0 explicit, 1 canonical implicit, ORC_IMPLICIT_BACKUP the
+ * alternative pairing */
+#define ORC_IMPLICIT_BACKUP 2
 void orc_fpi_solver(orc_state *s, int t, int implicit_mode); /* fpi_solver.f90:20-45 */
 void orc_run(orc_state *s, int t0, int n_iters, int implicit_mode,
              double *res_new_hist, double *residue_hist);
@@ -88,6 +95,7 @@ void orc_flux_octant(int axis, int sign_a, int sign_n, const double t1[3],
 void orc_venkat_limiter(const orc_state *s, const double qtilde[5], double phi[5], int k0);
 void orc_qtilde_to_primitive(const double q[5], double prim[5]);
 double orc_implicit_D_inv(const orc_state *s, int i0);
+double orc_implicit_D_inv_outer(const orc_state *s, int i0); /* implicit_D_inv_outer.f90:17-363 */
 void orc_conserved_vector_Ubar(const orc_state *s, int k0, double Ubar[5]);
 
 #ifdef __cplusplus

@@ -68,7 +68,8 @@ def lib():
         for name in ["orc_classify_points", "orc_generate_split_stencils", "orc_aliasing",
                      "orc_compute_conserved_vector", "orc_eval_q_variables",
                      "orc_eval_q_derivatives", "orc_timestep_delt", "orc_eval_flux_residual",
-                     "orc_implicit_sweeps", "orc_explicit_sweep", "orc_state_update"]:
+                     "orc_implicit_sweeps", "orc_explicit_sweep", "orc_state_update",
+                     "orc_implicit_sweeps_backup", "orc_state_update_implicit"]:
             getattr(L, name).argtypes = [sp]
             getattr(L, name).restype = None
         L.orc_set_flow.argtypes = [sp, C.c_double, C.c_double]
@@ -80,9 +81,24 @@ def lib():
         L.orc_qtilde_to_primitive.argtypes = [_dp, _dp]
         L.orc_implicit_D_inv.argtypes = [sp, C.c_int]
         L.orc_implicit_D_inv.restype = C.c_double
+        L.orc_implicit_D_inv_outer.argtypes = [sp, C.c_int]
+        L.orc_implicit_D_inv_outer.restype = C.c_double
         L.orc_conserved_vector_Ubar.argtypes = [sp, C.c_int, _dp]
         _lib = L
     return _lib
+
+
+# iteration forms: the explicit form, the canonical implicit pairing
+# (implicit_forward_and_reverse_sweeps.f90 + state_update.f90) and the
+# alternative one (..._backup.f90 + state_update_implicit.f90)
+MODE_CODES = {"explicit": 0, "implicit": 1, "implicit_backup": 2}
+
+
+def mode_code(mode):
+    """bool (implicit or not) or a mode name -> the oracle's implicit_mode"""
+    if isinstance(mode, str):
+        return MODE_CODES[mode]
+    return 1 if mode else 0
 
 
 def _dptr(a):
@@ -155,13 +171,14 @@ class Oracle:
     def call(self, name):
         getattr(lib(), "orc_" + name)(C.byref(self.s))
 
-    def fpi_solver(self, t, implicit):
-        lib().orc_fpi_solver(C.byref(self.s), t, 1 if implicit else 0)
+    def fpi_solver(self, t, mode):
+        """mode: True/False (canonical implicit / explicit) or a mode name"""
+        lib().orc_fpi_solver(C.byref(self.s), t, mode_code(mode))
 
-    def run(self, t0, n_iters, implicit):
+    def run(self, t0, n_iters, mode):
         rn = np.zeros(n_iters)
         rs = np.zeros(n_iters)
-        lib().orc_run(C.byref(self.s), t0, n_iters, 1 if implicit else 0, _dptr(rn), _dptr(rs))
+        lib().orc_run(C.byref(self.s), t0, n_iters, mode_code(mode), _dptr(rn), _dptr(rs))
         return rn, rs
 
     def split_lists(self):
@@ -212,10 +229,13 @@ def qtilde_to_primitive(q):
 REF_DIR = os.path.join(HERE, "_ref")
 
 
-def build_reference(tag, n, iters, mach, aoa, cfl, mode):
+def build_reference(tag, n, iters, mach, aoa, cfl, mode, power=0.0, inner_iterations=0):
+    """mode: explicit, implicit (canonical) or implicit_backup (the
+    type-ordered sweep with state_update_implicit.f90)."""
     exe = os.path.join(REF_DIR, tag, "ref_driver")
     subprocess.run([os.path.join(HERE, "build_ref.sh"), tag, str(n), str(iters), repr(mach),
-                    repr(aoa), repr(cfl), mode], check=True, capture_output=True)
+                    repr(aoa), repr(cfl), mode, repr(float(power)), str(int(inner_iterations))],
+                   check=True, capture_output=True)
     return exe
 
 
@@ -230,14 +250,15 @@ def _pointcloud():
 
 
 def run_reference(exe, cloud, prim, iters, stepwise=True, workdir=None, env=None,
-                  no_aslr=True, attempts=2):
-    """Run the reference driver on a cloud; return the parsed dumps."""
+                  no_aslr=True, attempts=2, trace=False):
+    """Run the reference driver on a cloud; return the parsed dumps.  With
+    trace, also the per-iteration far-field pressure record (trace.bin)."""
     pc = _pointcloud()
     own = workdir is None
     wd = workdir or tempfile.mkdtemp(prefix="mfcfd_ref_")
     pc.write_input_deck(os.path.join(wd, "3d_input_data"), cloud)
     pc.write_solution(os.path.join(wd, "stored-solution.dat"), cloud, prim)
-    cmd = [exe, str(iters), wd, "1" if stepwise else "0"]
+    cmd = [exe, str(iters), wd, "1" if stepwise else "0", "1" if trace else "0"]
     if no_aslr and shutil.which("setarch"):
         # a fixed address layout keeps the reference's out-of-range accesses
         # (alias entry 0, see build_ref.sh) landing in the same place
@@ -249,6 +270,8 @@ def run_reference(exe, cloud, prim, iters, stepwise=True, workdir=None, env=None
     else:
         raise RuntimeError(f"reference run failed ({r.returncode}): {r.stderr[-400:]!r}")
     out = parse_reference_dumps(wd, cloud.n, iters, stepwise)
+    if trace:
+        out["trace"] = np.fromfile(os.path.join(wd, "trace.bin"), dtype=np.float64).reshape(iters, 3)
     if own:
         shutil.rmtree(wd, ignore_errors=True)
     return out

@@ -12,9 +12,12 @@
 !              U, q, qm, dq, delt, flux_res, delUp, delUn, prim, delUp, res_new
 !   final.bin  prim, U, q, qm, dq, delt, flux_res, delUp, delUn
 !   time.txt   seconds spent in the timed fpi_solver loop, iterations timed
+!   trace.bin  (trace = 1 only) per iteration: the largest |p - pr_inf| over
+!              the outer points, the smallest pressure over all points and
+!              the number of points with a non-finite pressure (float64)
 !
 ! Usage (in a directory holding 3d_input_data and stored-solution.dat):
-!   ref_driver <iterations> <outdir> [stepwise(0|1)]
+!   ref_driver <iterations> <outdir> [stepwise(0|1) [trace(0|1)]]
 program ref_driver
     use data_structure_mod
     use point_preprocessor_mod
@@ -30,10 +33,11 @@ program ref_driver
     use implicit_forward_and_reverse_sweeps_mod
     use state_update_mod
     implicit none
-    integer :: t, niter, stepwise, ntimed, u
+    integer :: t, niter, stepwise, ntimed, u, trace, m, k
     integer(kind=8) :: c0, c1, crate
     character(len=512) :: arg, outdir
-    real*8, allocatable :: hist(:,:)
+    real*8, allocatable :: hist(:,:), tr(:,:)
+    real*8 :: pk
 
     call get_command_argument(1, arg)
     read(arg, *) niter
@@ -43,8 +47,14 @@ program ref_driver
         call get_command_argument(3, arg)
         read(arg, *) stepwise
     endif
-    allocate(hist(3, max(niter, 1)))
+    trace = 0
+    if (command_argument_count() >= 4) then
+        call get_command_argument(4, arg)
+        read(arg, *) trace
+    endif
+    allocate(hist(3, max(niter, 1)), tr(3, max(niter, 1)))
     hist = 0.0d0
+    tr = 0.0d0
 
     call allocate_size()
     call read_input_point_data()
@@ -68,6 +78,17 @@ program ref_driver
         hist(1, t) = res_new
         hist(2, t) = residue
         hist(3, t) = max_res
+        if (trace /= 0) then
+            do m = 1, outer_points
+                k = outer_points_index(m)
+                tr(1, t) = max(tr(1, t), abs(point%prim(5, k) - pr_inf))
+            enddo
+            tr(2, t) = minval(point%prim(5, :))
+            do k = 1, max_points
+                pk = point%prim(5, k)
+                if (.not. (pk == pk) .or. abs(pk) > huge(pk)) tr(3, t) = tr(3, t) + 1.0d0
+            enddo
+        endif
     enddo
     call system_clock(c1)
     open(newunit=u, file=trim(outdir)//'/time.txt', status='replace')
@@ -106,6 +127,11 @@ program ref_driver
         hist(3, t) = max_res
     endif
 
+    if (trace /= 0) then
+        open(newunit=u, file=trim(outdir)//'/trace.bin', form='unformatted', access='stream', status='replace')
+        write(u) tr(:, 1:niter)
+        close(u)
+    endif
     open(newunit=u, file=trim(outdir)//'/hist.bin', form='unformatted', access='stream', status='replace')
     write(u) hist(:, 1:niter)
     close(u)

@@ -36,7 +36,22 @@ def load_fixture(name):
     return cl, d
 
 
+# the canonical configurations (power 0, no inner iterations, statuses 0/1/2)
 FIXTURE_NAMES = ["explicit_blocks", "explicit_morton", "implicit_morton"]
+# branches the canonical configurations never reach: supersonic points
+# (status 5/6), power = 2 with one inner q-derivative iteration
+BRANCH_FIXTURES = ["explicit_supersonic", "implicit_supersonic", "explicit_power2_inner1",
+                   "implicit_power2_inner1"]
+# the alternative implicit pairing (..._backup.f90 + state_update_implicit.f90)
+BACKUP_FIXTURES = ["implicit_backup"]
+ALL_FIXTURES = FIXTURE_NAMES + BRANCH_FIXTURES + BACKUP_FIXTURES
+
+
+def fixture_params(d):
+    """(power, inner_iterations) a fixture was written with"""
+    power = float(d["power"]) if "power" in d else 0.0
+    inner = int(d["inner_iterations"]) if "inner_iterations" in d else 0
+    return power, inner
 
 
 @pytest.fixture(scope="session")

@@ -24,9 +24,24 @@ would overwrite the tail of the previously allocated array.  The fixture
 program (ref_driver) therefore runs with glibc.malloc.mmap_threshold=4096 so small clouds are laid
 out like large ones.
 
+Beyond the three canonical fixtures, four pin branches that the canonical
+configurations never reach: supersonic inlet/outlet points (status 5/6,
+state_update.f90:80-114 and implicit_forward_and_reverse_sweeps.f90:38-52),
+the WLS weight 1/dist**power with power = 2 together with one second-order
+inner iteration of the q-derivatives (parameter.f90:21,24,
+q_derivatives.f90:116-201), in both iteration forms; and the alternative
+implicit pairing (implicit_forward_and_reverse_sweeps_backup.f90 with
+state_update_implicit.f90), which starts from a perturbed state because a
+freestream start never evolves under it.
+
 --c1 additionally runs the 50k-point explicit configuration for 1000
 iterations (about 20 minutes on one core) and stores its residual history
 and final wall state.
+
+--cfl50 runs the canonical implicit form at CFL 50 on a 20k-point shell
+cloud (far field at radius 15) until its pressures turn non-finite and
+stores the residual history and the far-field pressure trace of every
+iteration (about 10 minutes on one core).
 """
 
 from __future__ import annotations
@@ -51,14 +66,61 @@ import orc  # noqa: E402
 pc = importlib.import_module("3d-mfcfd_amd").pointcloud
 
 FIXTURES = {
-    # name: (cloud kwargs, mode, mach, aoa, cfl, iterations)
+    # name: (cloud kwargs, mode, mach, aoa, cfl, iterations[, extras])
+    # extras: power, inner_iterations, supersonic (relabel far-field points
+    # 5/6), perturb (start from a perturbed freestream)
     "explicit_blocks": (dict(n_interior=600, axes=(0.5, 0.3, 0.2), seed=11, order="blocks"),
                         "explicit", 0.6, 2.0, 0.2, 40),
     "explicit_morton": (dict(n_interior=1200, axes=(0.5, 0.167, 0.05), seed=12, order="morton"),
                         "explicit", 0.84, 3.06, 0.2, 30),
     "implicit_morton": (dict(n_interior=1200, axes=(0.5, 0.167, 0.05), seed=11, order="morton"),
                         "implicit", 0.84, 3.06, 2.0, 30),
+    "explicit_supersonic": (dict(n_interior=600, axes=(0.5, 0.3, 0.2), seed=13, order="blocks"),
+                            "explicit", 0.6, 2.0, 0.2, 30, dict(supersonic=True)),
+    "implicit_supersonic": (dict(n_interior=1200, axes=(0.5, 0.167, 0.05), seed=14, order="morton"),
+                            "implicit", 0.84, 3.06, 2.0, 30, dict(supersonic=True)),
+    "explicit_power2_inner1": (dict(n_interior=600, axes=(0.5, 0.3, 0.2), seed=15, order="blocks"),
+                               "explicit", 0.6, 2.0, 0.2, 30, dict(power=2.0, inner_iterations=1)),
+    "implicit_power2_inner1": (dict(n_interior=1200, axes=(0.5, 0.167, 0.05), seed=16, order="morton", k=22),
+                               "implicit", 0.84, 3.06, 2.0, 30, dict(power=2.0, inner_iterations=1)),
+    # the alternative pairing turns outer points non-finite from iteration 2
+    # (implicit_D_inv_outer.f90:360 divides by a tan2 offset, not the time
+    # step, and goes negative), so the fixture holds two iterations
+    "implicit_backup": (dict(n_interior=1200, axes=(0.5, 0.167, 0.05), seed=17, order="morton"),
+                        "implicit_backup", 0.84, 3.06, 2.0, 2, dict(supersonic=True, perturb=True)),
 }
+
+# CFL 50 on a shell cloud: wall 1600, outer 800, interior 17600
+CFL50 = dict(n=20000, seed=7, mach=0.84, aoa=3.06, cfl=50.0, max_iters=200)
+
+
+def relabel_supersonic(cl):
+    """Far-field points upstream (x < -0.2 R) become supersonic inlet points
+    (status 5), downstream (x > 0.2 R) supersonic outlet points (6), every
+    second one of each, so outer points of status 2 remain on both sides.
+    Point 1 keeps its status."""
+    st = cl.status.copy()
+    outer = np.flatnonzero(st == pc.STATUS_OUTER)
+    outer = outer[outer != 0]
+    r = np.sqrt(cl.x[outer] ** 2 + cl.y[outer] ** 2 + cl.z[outer] ** 2)
+    up = outer[(cl.x[outer] < -0.2 * r)][::2]
+    down = outer[(cl.x[outer] > 0.2 * r)][::2]
+    st[up] = pc.STATUS_SUPERSONIC_INLET
+    st[down] = pc.STATUS_SUPERSONIC_OUTLET
+    cl.status = st
+    return cl
+
+
+def perturbed_start(cl, mach, aoa):
+    """Freestream times a smooth field of relative amplitude 1%."""
+    prim = np.tile(pc.freestream_prim(mach, aoa), (cl.n, 1))
+    f = 0.01 * np.sin(3.0 * cl.x + 1.0) * np.cos(2.0 * cl.y - 0.5) * np.cos(5.0 * cl.z)
+    prim[:, 0] *= 1.0 + f
+    prim[:, 1] *= 1.0 - 0.5 * f
+    prim[:, 2] += 0.02 * f
+    prim[:, 3] += 0.01 * f
+    prim[:, 4] *= 1.0 + 1.4 * f
+    return prim
 
 ENV = dict(os.environ, GLIBC_TUNABLES="glibc.malloc.mmap_threshold=4096")
 
@@ -70,15 +132,31 @@ def cloud_digest(cl) -> str:
     return h.hexdigest()
 
 
+def fixture_spec(name):
+    spec = FIXTURES[name]
+    kw, mode, mach, aoa, cfl, iters = spec[:6]
+    extra = dict(power=0.0, inner_iterations=0, supersonic=False, perturb=False)
+    if len(spec) > 6:
+        extra.update(spec[6])
+    return kw, mode, mach, aoa, cfl, iters, extra
+
+
 def make_fixture(name):
-    kw, mode, mach, aoa, cfl, iters = FIXTURES[name]
+    kw, mode, mach, aoa, cfl, iters, ex = fixture_spec(name)
     cl = pc.fixture_cloud(**kw)
-    exe = orc.build_reference(f"golden_{name}", cl.n, 1000, mach, aoa, cfl, mode)
-    prim0 = np.tile(pc.freestream_prim(mach, aoa), (cl.n, 1))
+    if ex["supersonic"]:
+        cl = relabel_supersonic(cl)
+    exe = orc.build_reference(f"golden_{name}", cl.n, 1000, mach, aoa, cfl, mode,
+                              ex["power"], ex["inner_iterations"])
+    if ex["perturb"]:
+        prim0 = perturbed_start(cl, mach, aoa)
+    else:
+        prim0 = np.tile(pc.freestream_prim(mach, aoa), (cl.n, 1))
     ref = orc.run_reference(exe, cl, prim0, iters, stepwise=True, env=ENV)
     out = dict(x=cl.x, y=cl.y, z=cl.z, tan1=cl.tan1, tan2=cl.tan2, nor=cl.nor,
                status=cl.status, nbhs=cl.nbhs, conn=cl.conn, min_dist=cl.min_dist,
                prim0=prim0, mode=np.array(mode), mach=mach, aoa=aoa, cfl=cfl,
+               power=float(ex["power"]), inner_iterations=int(ex["inner_iterations"]),
                iterations=iters, digest=np.array(cloud_digest(cl)))
     for key in ("split_counts", "split_lists", "alias", "point_with_alias", "hist",
                 "start_prim", "start_delUp", "start_delUn", "start_res_old",
@@ -108,17 +186,47 @@ def make_c1():
     print(f"c1_50k: {time.time() - t0:.0f} s -> {path}")
 
 
+def cfl50_cloud():
+    c = CFL50
+    n_wall, n_outer = int(0.08 * c["n"]), int(0.04 * c["n"])
+    return pc.ellipsoid_cloud(n_wall, n_outer, c["n"] - n_wall - n_outer, seed=c["seed"], order="morton")
+
+
+def make_cfl50():
+    """Canonical implicit form at CFL 50 until the first non-finite
+    pressure: residual history and far-field trace of every iteration."""
+    c = CFL50
+    cl = cfl50_cloud()
+    exe = orc.build_reference("cfl50_20k", cl.n, c["max_iters"], c["mach"], c["aoa"], c["cfl"], "implicit")
+    prim0 = np.tile(pc.freestream_prim(c["mach"], c["aoa"]), (cl.n, 1))
+    t0 = time.time()
+    ref = orc.run_reference(exe, cl, prim0, c["max_iters"], stepwise=False, env=ENV, trace=True)
+    tr = ref["trace"]
+    bad = np.flatnonzero(tr[:, 2] > 0)
+    first_nonfinite = int(bad[0]) + 1 if bad.size else 0
+    last = first_nonfinite if first_nonfinite else c["max_iters"]
+    path = os.path.join(HERE, "cfl50_20k_history.npz")
+    np.savez_compressed(path, hist=ref["hist"][:last], trace=tr[:last], digest=np.array(cloud_digest(cl)),
+                        first_nonfinite=first_nonfinite, mach=c["mach"], aoa=c["aoa"], cfl=c["cfl"],
+                        n=c["n"], seed=c["seed"], seconds=ref["seconds"], timed_iters=ref["timed_iters"])
+    print(f"cfl50_20k: first non-finite iteration {first_nonfinite}, {time.time() - t0:.0f} s -> {path}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--c1", action="store_true", help="also run the 50k x 1000 history")
+    ap.add_argument("--cfl50", action="store_true", help="also run the CFL-50 history on 20k points")
     ap.add_argument("--only", default=None)
+    ap.add_argument("--skip-fixtures", action="store_true")
     args = ap.parse_args()
     for name in FIXTURES:
-        if args.only and name != args.only:
+        if args.skip_fixtures or (args.only and name != args.only):
             continue
         make_fixture(name)
     if args.c1:
         make_c1()
+    if args.cfl50:
+        make_cfl50()
 
 
 if __name__ == "__main__":

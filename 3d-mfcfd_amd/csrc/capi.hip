@@ -147,16 +147,37 @@ struct mfcfd_ctx {
     DevBuf<ResidualRecord> rec;
     DevBuf<double> hist;
     ResidualRecord* host_rec = nullptr;  // pinned
-    // sweep schedule (device point numbers)
-    Schedule sched;
-    std::vector<int> fwd_ptr, bwd_ptr;
-    DevBuf<int> fwd_points, bwd_points;
-    // sync-free sweeps: points in execution order, the list entries each
-    // position takes, counters
-    DevBuf<int> fwd_seq, bwd_seq;
-    DevBuf<long long> fwd_take_ptr, bwd_take_ptr;
-    DevBuf<unsigned> fwd_take, bwd_take;
+    // LU-SGS sweep plan of one sweep direction on the device (SweepPlan):
+    // execution order, taken entries and their weights, snapshot and mark lists
+    struct DevSweepDir {
+        DevBuf<int> seq, old_pts, old_src, wait_src;
+        DevBuf<long long> take_ptr;
+        DevBuf<unsigned> take;
+        DevBuf<double> wgt;
+        std::vector<int> level_ptr;   // host copy of the level cuts of seq
+        int count = 0;
+        void release() {
+            seq.release();
+            old_pts.release();
+            old_src.release();
+            wait_src.release();
+            take_ptr.release();
+            take.release();
+            wgt.release();
+            level_ptr.clear();
+            count = 0;
+        }
+    };
+    DevSweepDir sw_fwd, sw_bwd;
+    int plan_form = -1;    // iteration form the plan was built for (-1: none)
+    DevBuf<double> pub, pold, dold;   // sweep values: published, before the sweep (prim / delU)
     DevBuf<unsigned> sweep_ctr;
+    // host inputs of plan building (device order), kept from bind
+    std::vector<int> h_status, h_alias, h_nbhs, h_conn, h_cnt, h_lst;
+    std::vector<long long> h_off;
+    std::vector<double> h_x, h_y, h_z;
+    std::vector<int> h_pwa;                              // point_with_alias as device numbers, -1 empty
+    std::vector<int> h_wall, h_int, h_out, h_sin, h_sout;  // class lists, host order, device numbers
     // clouds of a batch (mfcfd_bind_batch): instance of each device point,
     // freestream (u1_inf, u2_inf) per instance, device-order ranges; a single
     // cloud is one instance with batch = false
@@ -174,7 +195,6 @@ struct mfcfd_ctx {
     hipGraphExec_t graph_exec = nullptr;
     bool graph_stale = true;
     int sweep_grid = 0, sweep_grid_mixed = 0;
-    bool implicit_supported = true;
     // bookkeeping of fpi_solver.f90:38-43
     double res_old = 0.0, res_new = 0.0, residue = 0.0, max_res = 0.0, sum_res_sqr = 0.0;
     int max_res_point = 0, last_t = 0;
@@ -235,9 +255,14 @@ struct mfcfd_ctx {
         DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &nwalk,
                                &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
-                               &idx_fused_wall, &idx_generic, &partial_pos, &fwd_points, &bwd_points,
-                               &fwd_seq, &bwd_seq};
+                               &idx_fused_wall, &idx_generic, &partial_pos};
         for (auto* b : ints) b->release();
+        sw_fwd.release();
+        sw_bwd.release();
+        plan_form = -1;
+        pub.release();
+        pold.release();
+        dold.release();
         q32.release();
         qm32.release();
         dq32.release();
@@ -249,10 +274,6 @@ struct mfcfd_ctx {
         dir_off.release();
         walk.release();
         pos.release();
-        fwd_take_ptr.release();
-        bwd_take_ptr.release();
-        fwd_take.release();
-        bwd_take.release();
     }
     ~mfcfd_ctx() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -358,6 +379,30 @@ void collect_kernel_times(mfcfd_ctx* c) {
     c->tev_used = 0;
 }
 
+// the implicit iteration form of the current parameters
+int implicit_form(const mfcfd_ctx* c) { return c->params.mode == MFCFD_EXPLICIT ? MFCFD_IMPLICIT : c->params.mode; }
+
+SweepAsync sweep_args(mfcfd_ctx* c, mfcfd_ctx::DevSweepDir& d) {
+    SweepAsync a;
+    a.seq = d.seq.p;
+    a.count = d.count;
+    a.take_ptr = d.take_ptr.p;
+    a.take = d.take.p;
+    a.wgt = d.wgt.p;
+    a.old_pts = d.old_pts.p;
+    a.n_old = (int)d.old_pts.n;
+    a.old_src = d.old_src.p;
+    a.n_old_src = (int)d.old_src.n;
+    a.wait_src = d.wait_src.p;
+    a.n_wait_src = (int)d.wait_src.n;
+    a.pub = c->pub.p;
+    a.pold = c->pold.p;
+    a.dold = c->dold.p;
+    a.ctr = c->sweep_ctr.p;
+    a.stalls = &c->rec.p->stalls;
+    return a;
+}
+
 int do_sweeps(mfcfd_ctx* c) {
     const DevGeom g = c->geom();
     const DevState s = c->state();
@@ -365,34 +410,119 @@ int do_sweeps(mfcfd_ctx* c) {
         launch_explicit_sweep(g, s, c->stream);
         return launch_error("explicit sweep");
     }
-    if (!c->implicit_supported)
-        return fail(MFCFD_EUNSUPPORTED, "implicit sweeps with supersonic points are not supported");
+    if (c->plan_form != c->params.mode)
+        return fail(MFCFD_ESTATE, "no sweep plan for iteration form %d (internal: prepare_plan not called)",
+                    c->params.mode);
     launch_dinv(g, s, c->fp, c->tl, c->stream);
-    SweepSchedule sch;
-    sch.n_levels_fwd = c->sched.n_levels_fwd;
-    sch.n_levels_bwd = c->sched.n_levels_bwd;
-    sch.fwd_ptr = c->sched.fwd_ptr.data();
-    sch.bwd_ptr = c->sched.bwd_ptr.data();
-    sch.fwd_points = c->fwd_points.p;
-    sch.bwd_points = c->bwd_points.p;
-    sch.n_swept = (int)c->sched.order.size();
     if (c->params.sweep_schedule == MFCFD_SWEEP_LEVELS) {
+        SweepSchedule sch;
+        sch.n_levels_fwd = (int)c->sw_fwd.level_ptr.size() - 1;
+        sch.n_levels_bwd = (int)c->sw_bwd.level_ptr.size() - 1;
+        sch.fwd_ptr = c->sw_fwd.level_ptr.data();
+        sch.bwd_ptr = c->sw_bwd.level_ptr.data();
+        sch.fwd_points = c->sw_fwd.seq.p;
+        sch.bwd_points = c->sw_bwd.seq.p;
+        sch.n_swept = c->sw_fwd.count;
         launch_sweeps_levels(g, s, c->fp, sch, c->stream);
         return launch_error("implicit sweeps");
     }
-    SweepAsync fwd, bwd;
-    fwd.count = bwd.count = sch.n_swept;
-    fwd.ctr = bwd.ctr = c->sweep_ctr.p;
-    fwd.stalls = bwd.stalls = &c->rec.p->stalls;
-    fwd.seq = c->fwd_seq.p;
-    bwd.seq = c->bwd_seq.p;
-    fwd.take_ptr = c->fwd_take_ptr.p;
-    bwd.take_ptr = c->bwd_take_ptr.p;
-    fwd.take = c->fwd_take.p;
-    bwd.take = c->bwd_take.p;
+    const SweepAsync fwd = sweep_args(c, c->sw_fwd);
+    const SweepAsync bwd = sweep_args(c, c->sw_bwd);
     const int grid = c->fp.mixed ? c->sweep_grid_mixed : c->sweep_grid;
     launch_sweeps_async(g, s, c->fp, fwd, bwd, std::max(1, grid / std::max(1, c->shapes.sweep_div)), c->stream);
     return launch_error("implicit sweeps");
+}
+
+// Builds the LU-SGS plan of the given implicit form (SweepPlan) from the
+// host copies kept at bind, uploads it and computes every entry's weight.
+// Synchronous; never called while a graph is being captured.
+int build_plan(mfcfd_ctx* c, int form) {
+    const int n = c->n;
+    const bool backup = form == MFCFD_IMPLICIT_BACKUP;
+    SweepPlanInputs in;
+    in.n = n;
+    in.status = c->h_status.data();
+    in.alias = c->h_alias.data();
+    in.nbhs = c->h_nbhs.data();
+    in.conn = c->h_conn.data();
+    in.cnt = c->h_cnt.data();
+    in.off = c->h_off.data();
+    in.lst = c->h_lst.data();
+    in.x = c->h_x.data();
+    in.y = c->h_y.data();
+    in.z = c->h_z.data();
+    in.backup = backup;
+    if (!backup) {
+        // implicit_forward_and_reverse_sweeps.f90:26-85: point_with_alias
+        // order; wall, interior and supersonic points are visited, outer
+        // points and empty entries are not
+        for (int p : c->h_pwa) {
+            if (p < 0) continue;
+            const int st = c->h_status[p];
+            if (st == 0 || st == 1 || st == 5 || st == 6) in.visit_fwd.push_back(p);
+        }
+        in.visit_bwd.assign(in.visit_fwd.rbegin(), in.visit_fwd.rend());
+    } else {
+        // implicit_forward_and_reverse_sweeps_backup.f90:25-139: each class
+        // in index order, and in reverse for the backward sweep
+        const std::vector<int>* cls[5] = {&c->h_wall, &c->h_int, &c->h_out, &c->h_sin, &c->h_sout};
+        for (auto* v : cls) {
+            in.visit_fwd.insert(in.visit_fwd.end(), v->begin(), v->end());
+            in.visit_bwd.insert(in.visit_bwd.end(), v->rbegin(), v->rend());
+        }
+    }
+    SweepPlan plan;
+    if (plan.build(in)) return fail(MFCFD_EINVAL, "sweep plan: a dependency does not precede its point");
+    hipStream_t st = c->stream;
+    c->sw_fwd.release();
+    c->sw_bwd.release();
+    c->plan_form = -1;
+    int rc = 0;
+    auto put = [&](mfcfd_ctx::DevSweepDir& d, const SweepDirPlan& pd) -> int {
+        int r = 0;
+        d.count = (int)pd.seq.size();
+        d.level_ptr = pd.level_ptr;
+        const int none = 0;
+        const unsigned none_u = 0u;
+        if ((r = upload(d.seq, pd.seq.empty() ? &none : pd.seq.data(), std::max<size_t>(pd.seq.size(), 1), st)))
+            return r;
+        if ((r = upload(d.take_ptr, pd.take_ptr.data(), pd.take_ptr.size(), st))) return r;
+        if ((r = upload(d.take, pd.take.empty() ? &none_u : pd.take.data(), std::max<size_t>(pd.take.size(), 1), st)))
+            return r;
+        if (d.wgt.alloc(std::max<size_t>(pd.take.size(), 1)) != hipSuccess)
+            return fail(MFCFD_ENOMEM, "sweep weights (%zu entries)", pd.take.size());
+        if ((r = upload(d.old_pts, pd.old_pts.data(), pd.old_pts.size(), st))) return r;
+        if ((r = upload(d.old_src, pd.old_src.data(), pd.old_src.size(), st))) return r;
+        if ((r = upload(d.wait_src, pd.wait_src.data(), pd.wait_src.size(), st))) return r;
+        return MFCFD_OK;
+    };
+    if ((rc = put(c->sw_fwd, plan.fwd))) return rc;
+    if ((rc = put(c->sw_bwd, plan.bwd))) return rc;
+    if (c->pub.n == 0 && c->pub.alloc(5ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep values");
+    if (c->pold.n == 0 && c->pold.alloc(5ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep values");
+    const bool copies = !plan.fwd.old_src.empty() || !plan.bwd.old_src.empty();
+    if (copies && c->dold.n == 0 && c->dold.alloc(5ULL * n) != hipSuccess)
+        return fail(MFCFD_ENOMEM, "sweep values");
+    if (c->sweep_ctr.n == 0) {
+        if (c->sweep_ctr.alloc(2) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep counters");
+        HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, 2 * sizeof(unsigned), st));
+    }
+    launch_sweep_weights(c->geom(), sweep_args(c, c->sw_fwd), c->fp.power, st);
+    launch_sweep_weights(c->geom(), sweep_args(c, c->sw_bwd), c->fp.power, st);
+    if ((rc = launch_error("sweep weights"))) return rc;
+    HIP_TRY(hipStreamSynchronize(st));   // the host plan ends here
+    c->sweep_grid = sweep_async_grid(c->sw_fwd.count, c->params.power == 0.0, false);
+    c->sweep_grid_mixed = sweep_async_grid(c->sw_fwd.count, c->params.power == 0.0, true);
+    c->plan_form = form;
+    c->graph_stale = true;
+    return MFCFD_OK;
+}
+
+// every entry point that may run the implicit sweeps calls this first (and
+// outside any graph capture): the plan of the current iteration form
+int prepare_plan(mfcfd_ctx* c) {
+    if (c->params.mode == MFCFD_EXPLICIT || c->plan_form == c->params.mode) return MFCFD_OK;
+    return build_plan(c, c->params.mode);
 }
 
 int do_state_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr,
@@ -447,8 +577,14 @@ int fetch_record(mfcfd_ctx* c) {
     c->sum_res_sqr = c->host_rec->sum_res_sqr;
     c->max_res = c->host_rec->max_res;
     if (c->host_rec->max_res_point > 0) c->max_res_point = c->host_rec->max_res_point;
-    if (c->host_rec->stalls)
-        return fail(MFCFD_EHIP, "implicit sweep: %d dependency waits exceeded the spin limit", c->host_rec->stalls);
+    if (c->host_rec->stalls) {
+        // reported once: clear the count so later sweeps wait normally again
+        // (this runs outside any graph capture)
+        const int stalls = c->host_rec->stalls;
+        HIP_TRY(hipMemsetAsync(&c->rec.p->stalls, 0, sizeof(int), c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return fail(MFCFD_EHIP, "implicit sweep: %d dependency waits exceeded the spin limit", stalls);
+    }
     return MFCFD_OK;
 }
 
@@ -510,7 +646,8 @@ int mfcfd_destroy(mfcfd_ctx* ctx) {
 
 int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
     if (!c || !p) return fail(MFCFD_EINVAL, "null argument");
-    if (p->mode != MFCFD_EXPLICIT && p->mode != MFCFD_IMPLICIT) return fail(MFCFD_EINVAL, "mode %d", p->mode);
+    if (p->mode != MFCFD_EXPLICIT && p->mode != MFCFD_IMPLICIT && p->mode != MFCFD_IMPLICIT_BACKUP)
+        return fail(MFCFD_EINVAL, "mode %d", p->mode);
     if (p->precision != MFCFD_FP64 && p->precision != MFCFD_MIXED)
         return fail(MFCFD_EINVAL, "precision %d", p->precision);
     if (p->precision == MFCFD_MIXED) {
@@ -544,6 +681,7 @@ int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
     f.pr_inf = 1.0 / 1.4;
     f.inner_iterations = p->inner_iterations;
     f.mixed = p->precision == MFCFD_MIXED ? 1 : 0;
+    f.backup = p->mode == MFCFD_IMPLICIT_BACKUP ? 1 : 0;
     c->have_params = true;
     c->graph_stale = true;
     if (c->have_geom && p->inner_iterations > 0 && c->temp.n == 0) {
@@ -830,104 +968,46 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     if (c->partial.alloc(2ULL * c->n_partial) != hipSuccess || c->partial_pos.alloc(2ULL * c->n_partial) != hipSuccess)
         return fail(MFCFD_ENOMEM, "residual partials");
 
-    // sweep schedule over the alias order (built in host numbering)
-    std::vector<unsigned char> swept(n, 0);
+    // host inputs of the LU-SGS sweep plans (device order); a plan is built
+    // for the iteration form in use when the first implicit sweep needs it
+    c->h_status = status;
+    c->h_alias = alias;
+    c->h_nbhs = nbhs;
+    c->h_conn = conn0;
+    c->h_cnt = cnt;
+    c->h_off = off;
+    c->h_lst = lst;
+    c->h_x.resize(n);
+    c->h_y.resize(n);
+    c->h_z.resize(n);
+    for (int d = 0; d < n; ++d) {
+        c->h_x[d] = pos4[d].x;
+        c->h_y[d] = pos4[d].y;
+        c->h_z[d] = pos4[d].z;
+    }
+    c->h_pwa.assign(n, -1);
     for (int i = 0; i < n; ++i) {
         const int p = gm->point_with_alias[i];
-        if (p > 0 && (gm->status[p - 1] == 0 || gm->status[p - 1] == 1)) swept[p - 1] = 1;
+        if (p > 0) c->h_pwa[i] = inv[p - 1];
     }
-    c->implicit_supported = lsi.empty() && lso.empty();
-    Schedule::Inputs in;
-    in.n = n;
-    in.status = gm->status;
-    in.nbhs = gm->nbhs;
-    in.conn = gm->conn;
-    for (int d = 0; d < 6; ++d) {
-        in.split_cnt[d] = gm->split_nbhs[d];
-        in.split[d] = gm->split_conn[d];
-    }
-    in.alias = gm->alias;
-    in.pwa = gm->point_with_alias;
-    in.swept = swept.data();
-    c->sched.build(in);
-    std::vector<int> fwd(c->sched.fwd_points.size()), bwd(c->sched.bwd_points.size());
-    for (size_t e = 0; e < fwd.size(); ++e) fwd[e] = inv[c->sched.fwd_points[e]];
-    for (size_t e = 0; e < bwd.size(); ++e) bwd[e] = inv[c->sched.bwd_points[e]];
-    if ((rc = upload(c->fwd_points, fwd.data(), fwd.size(), st))) return rc;
-    if ((rc = upload(c->bwd_points, bwd.data(), bwd.size(), st))) return rc;
     {
-        // execution order of the sync-free sweeps: the points sorted by
-        // dependency level (the level schedule's order).  Any order in which
-        // every point follows its dependencies gives the same results; this
-        // one keeps the resident window of tickets on a few adjacent levels,
-        // where the sweep order (alias order) can hold long dependency chains.
-        const Schedule& S = c->sched;
-        const size_t m = S.order.size();
-        std::vector<int> seq_f(m), seq_b(m);
-        // sweep position t of the forward sweep is order[t], of the backward
-        // sweep order[m-1-t]
-        std::vector<int> at(n, -1);
-        for (size_t t = 0; t < m; ++t) at[S.order[t]] = (int)t;
-        // The sweep kernel waits, entry by entry, on exactly the dependencies
-        // the schedule lists; each must come earlier in the execution order,
-        // or a wait would never end.  Checked here, on the host.
-        auto build = [&](const std::vector<int>& lev_points, const std::vector<long long>& dptr,
-                         const std::vector<int>& dep, bool forward, std::vector<int>& seq) -> int {
-            std::vector<int> xpos(n, -1);
-            for (size_t u = 0; u < m; ++u) xpos[lev_points[u]] = (int)u;
-            for (size_t u = 0; u < m; ++u) {
-                const int p = lev_points[u];
-                seq[u] = inv[p];
-                const size_t t = forward ? (size_t)at[p] : m - 1 - (size_t)at[p];
-                for (long long e = dptr[t]; e < dptr[t + 1]; ++e)
-                    if (xpos[dep[e]] < 0 || xpos[dep[e]] >= (int)u) return -1;
+        // class lists in host index order (point_preprocessor.f90:41-95)
+        c->h_wall.clear();
+        c->h_int.clear();
+        c->h_out.clear();
+        c->h_sin.clear();
+        c->h_sout.clear();
+        for (int r = 0; r < n; ++r) {
+            switch (gm->status[r]) {
+                case 0: c->h_int.push_back(inv[r]); break;
+                case 1: c->h_wall.push_back(inv[r]); break;
+                case 2: c->h_out.push_back(inv[r]); break;
+                case 5: c->h_sin.push_back(inv[r]); break;
+                case 6: c->h_sout.push_back(inv[r]); break;
             }
-            return 0;
-        };
-        if (build(S.fwd_points, S.fwd_dep_ptr, S.fwd_dep, true, seq_f))
-            return fail(MFCFD_EINVAL, "forward sweep dependency out of order");
-        if (build(S.bwd_points, S.bwd_dep_ptr, S.bwd_dep, false, seq_b))
-            return fail(MFCFD_EINVAL, "backward sweep dependency out of order");
-        if ((rc = upload(c->fwd_seq, seq_f.data(), m, st))) return rc;
-        if ((rc = upload(c->bwd_seq, seq_b.data(), m, st))) return rc;
-        // the entries of each position's direction lists that its sweep
-        // takes: neighbours with a smaller alias (forward) or a larger one
-        // (backward), every list but a wall point's ZPOS (its z term walks
-        // conn, the ZNEG list), each packed as k << 3 | direction
-        auto takes = [&](const std::vector<int>& seq, bool forward, DevBuf<long long>& dptr,
-                         DevBuf<unsigned>& dent) -> int {
-            std::vector<long long> ptr(m + 1, 0);
-            std::vector<unsigned> ent;
-            ent.reserve(m * 40);
-            for (size_t u = 0; u < m; ++u) {
-                const int d = seq[u];
-                const int ad = alias[d];
-                for (int dir = 0; dir < 6; ++dir) {
-                    if (status[d] == 1 && dir == ZPOS) continue;
-                    const int cn = cnt[(size_t)dir * n + d];
-                    const long long o = off[(size_t)dir * n + d];
-                    for (int j = 0; j < cn; ++j) {
-                        const int k = lst[o + j];
-                        if (forward ? alias[k] < ad : alias[k] > ad) ent.push_back(((unsigned)k << 3) | (unsigned)dir);
-                    }
-                }
-                ptr[u + 1] = (long long)ent.size();
-            }
-            int r2 = upload(dptr, ptr.data(), ptr.size(), st);
-            if (r2) return r2;
-            if (ent.empty()) ent.push_back(0u);  // keep the buffer allocated
-            if ((r2 = upload(dent, ent.data(), ent.size(), st))) return r2;
-            HIP_TRY(hipStreamSynchronize(st));  // the host vectors end here
-            return MFCFD_OK;
-        };
-        if ((rc = takes(seq_f, true, c->fwd_take_ptr, c->fwd_take))) return rc;
-        if ((rc = takes(seq_b, false, c->bwd_take_ptr, c->bwd_take))) return rc;
-        if (c->sweep_ctr.alloc(2) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep counters");
-        HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, 2 * sizeof(unsigned), st));
-        HIP_TRY(hipMemsetAsync(c->rec.p, 0, sizeof(ResidualRecord), st));
-        c->sweep_grid = sweep_async_grid((int)m, c->params.power == 0.0, false);
-        c->sweep_grid_mixed = sweep_async_grid((int)m, c->params.power == 0.0, true);
+        }
     }
+    HIP_TRY(hipMemsetAsync(c->rec.p, 0, sizeof(ResidualRecord), st));
     // instances: device-order ranges (one for a single cloud), per-point
     // instance and freestream table for a batch
     {
@@ -976,6 +1056,9 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
             return fail(MFCFD_ENOMEM, "instance residual buffers");
         HIP_TRY(hipMemsetAsync(c->ressq.p, 0, (size_t)n * sizeof(double), st));
         HIP_TRY(hipMemsetAsync(c->hist_ctr.p, 0, sizeof(int), st));
+        // the pageable host vectors of this block end here: their copies must
+        // have completed
+        HIP_TRY(hipStreamSynchronize(st));
     }
     HIP_TRY(hipStreamSynchronize(st));
     c->have_geom = true;
@@ -1193,6 +1276,7 @@ int mfcfd_eval_flux_residual(mfcfd_ctx* c) {
 int mfcfd_implicit_forward_backward_sweeps(mfcfd_ctx* c) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
+    if ((rc = prepare_plan(c))) return rc;
     return do_sweeps(c);
 }
 
@@ -1206,6 +1290,7 @@ int mfcfd_state_update(mfcfd_ctx* c) {
 int mfcfd_fpi_solver(mfcfd_ctx* c, int t, double* res_new, double* residue) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
+    if ((rc = prepare_plan(c))) return rc;
     if ((rc = do_iteration(c, nullptr, 0))) return rc;
     if ((rc = fetch_record(c))) return rc;
     residue_bookkeeping(c, t, c->res_new);
@@ -1219,6 +1304,7 @@ int mfcfd_run(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, double* r
     if (rc) return rc;
     if (n_iters < 0) return fail(MFCFD_EINVAL, "n_iters < 0");
     if (n_iters == 0) return MFCFD_OK;
+    if ((rc = prepare_plan(c))) return rc;
     if (c->hist.n < (size_t)n_iters) {
         if (c->hist.alloc((size_t)n_iters) != hipSuccess) return fail(MFCFD_ENOMEM, "history buffer");
         c->graph_stale = true;
@@ -1242,6 +1328,8 @@ int mfcfd_run_graph(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, dou
     if (n_iters < 0) return fail(MFCFD_EINVAL, "n_iters < 0");
     if (n_iters == 0) return MFCFD_OK;
     if (c->time_flux) return fail(MFCFD_ESTATE, "kernel timing records events; disable it before graph runs");
+    // plan building allocates and copies: never inside the capture
+    if ((rc = prepare_plan(c))) return rc;
     // the history buffers' addresses are captured: grow them before capture
     const size_t need = (size_t)std::max(n_iters, 64);
     if (c->hist.n < need) {
@@ -1371,6 +1459,7 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
         return out;
     };
     mfcfd_tuning t{};
+    if ((rc = prepare_plan(c))) return rc;
     // inputs of the timed kernels: the first steps of an iteration from prim
     launch_conserved_and_q(g, s, c->fp, c->stream);
     // predecessors in the iteration: the q-variables before the q-derivatives,
@@ -1462,8 +1551,9 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
     // the sweeps are a function of flux_res, prim, U and delt only (each
     // point reads its dependencies' values of the same sweep), so repeating
     // them leaves the state as one sweep would
-    if (c->params.mode == MFCFD_IMPLICIT && c->implicit_supported &&
-        c->params.sweep_schedule == MFCFD_SWEEP_ASYNC) {
+    // (the alternative driver's old entries read values of the previous
+    // repeat, so its sweeps are not tuned)
+    if (c->params.mode == MFCFD_IMPLICIT && c->params.sweep_schedule == MFCFD_SWEEP_ASYNC) {
         const int divs[3] = {1, 2, 4};
         double best_ms = 1e30;
         for (int dv : divs) {
@@ -1534,6 +1624,7 @@ int mfcfd_time_phases(mfcfd_ctx* c, int t0, int n_iters, double* out_ms) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
     if (n_iters <= 0 || !out_ms) return fail(MFCFD_EINVAL, "n_iters must be positive and out_ms non-null");
+    if ((rc = prepare_plan(c))) return rc;
     hipEvent_t ev[MFCFD_PHASE_COUNT + 1];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     double acc[MFCFD_PHASE_COUNT] = {0, 0, 0, 0, 0, 0};
@@ -1585,10 +1676,14 @@ int mfcfd_time_phases(mfcfd_ctx* c, int t0, int n_iters, double* out_ms) {
 }
 
 int mfcfd_sweep_info(mfcfd_ctx* c, int* levels_fwd, int* levels_bwd, int* n_swept) {
-    if (!c || !c->have_geom) return fail(MFCFD_ESTATE, "no geometry bound");
-    if (levels_fwd) *levels_fwd = c->sched.n_levels_fwd;
-    if (levels_bwd) *levels_bwd = c->sched.n_levels_bwd;
-    if (n_swept) *n_swept = (int)c->sched.order.size();
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    // the plan of the implicit form in use (the canonical one in explicit mode)
+    const int form = implicit_form(c);
+    if (c->plan_form != form && (rc = build_plan(c, form))) return rc;
+    if (levels_fwd) *levels_fwd = (int)c->sw_fwd.level_ptr.size() - 1;
+    if (levels_bwd) *levels_bwd = (int)c->sw_bwd.level_ptr.size() - 1;
+    if (n_swept) *n_swept = c->sw_fwd.count;
     return MFCFD_OK;
 }
 

@@ -113,12 +113,61 @@ __device__ __forceinline__ Frame<double> load_frame(const double* frame, int k) 
     return f;
 }
 
+// dist**power as the reference build evaluates it: power is a compile-time
+// PARAMETER (parameter.f90:21), and the constant exponents 0, 0.5, 1, 2 and
+// 3 reduce to 1, sqrt(d), d, d*d and (d*d)*d (flang's x86 code of the
+// reference); any other exponent is a pow call.  pow(d, 2) need not equal
+// d*d rounded once, so the reduction is kept bit for bit.
+__device__ __forceinline__ double ref_pow(double d, double power) {
+    if (power == 0.0) return 1.0;
+    if (power == 0.5) return __builtin_sqrt(d);
+    if (power == 1.0) return d;
+    if (power == 2.0) return d * d;
+    if (power == 3.0) return d * d * d;
+    return pow(d, power);
+}
+
 // least-squares weight 1/dist**power; UNIT_W selects the power = 0 kernels,
 // where the weight is exactly 1 and no power function is compiled in
 template <bool UNIT_W>
 __device__ __forceinline__ double wls_weight(double a, double b, double c, double power) {
     if (UNIT_W) return 1.0;
-    return 1.0 / pow(__builtin_sqrt(a * a + b * b + c * c), power);
+    return 1.0 / ref_pow(__builtin_sqrt(a * a + b * b + c * c), power);
+}
+
+// Cramer weights of one direction list of point i: the geometric sums of the
+// list in list order (interior_flux_dGxpos.f90:84-90) and the cofactors of
+// column col (the list's axis) divided by the determinant, so that the
+// list's derivative is sum_j (G_k - G_i) * (c0*dels_j + c1*delt_j + c2*deln_j)
+// times the weight of entry j.  An empty list gives zeros.
+template <bool UW>
+__device__ __forceinline__ void list_cofactors(const DevGeom& g, int i, const Frame<double>& f, const int* list,
+                                               int cnt, int col, double power, double c[3]) {
+    const double4 pi4 = g.pos[i];
+    Geo6<double> geo = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < cnt; ++j) {
+        const double4 pk = g.pos[list[j]];
+        const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
+        const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+        const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+        const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+        const double w = wls_weight<UW>(dels, delt, deln, power);
+        const double dels_w = dels * w, delt_w = delt * w, deln_w = deln * w;
+        geo.xx = geo.xx + dels * dels_w;
+        geo.yy = geo.yy + delt * delt_w;
+        geo.zz = geo.zz + deln * deln_w;
+        geo.xy = geo.xy + dels * delt_w;
+        geo.yz = geo.yz + delt * deln_w;
+        geo.zx = geo.zx + deln * dels_w;
+    }
+    if (cnt == 0) {
+        c[0] = c[1] = c[2] = 0.0;
+        return;
+    }
+    const double det = cramer_det(geo);
+    c[0] = cramer_num(geo, col, 1.0, 0.0, 0.0) / det;
+    c[1] = cramer_num(geo, col, 0.0, 1.0, 0.0) / det;
+    c[2] = cramer_num(geo, col, 0.0, 0.0, 1.0) / det;
 }
 
 // Limited reconstruction of q at one end of an edge and conversion to

@@ -122,6 +122,8 @@ struct FlowParams {
     double rho_inf, u1_inf, u2_inf, u3_inf, pr_inf;
     int inner_iterations;
     int mixed;   // 1: q, qm, dq and the split fluxes in FP32
+    int backup;  // 1: the alternative implicit pairing (implicit_forward_and_reverse_sweeps_backup.f90
+                 // with state_update_implicit.f90)
 };
 
 // device-side residual record, state_update.f90:115 and data_structure.f90:58-59

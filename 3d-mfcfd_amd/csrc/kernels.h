@@ -36,6 +36,8 @@ struct LaunchShapes {
     int sweep_div = 1;   // sync-free sweeps use the resident grid / sweep_div
 };
 
+// level-set schedule of both sweeps: the plan's execution orders cut at
+// their dependency levels (one launch per level)
 struct SweepSchedule {
     int n_levels_fwd, n_levels_bwd;
     const int* fwd_ptr;      // host copy: level l covers fwd_points[fwd_ptr[l] .. fwd_ptr[l+1])
@@ -45,12 +47,23 @@ struct SweepSchedule {
     int n_swept;
 };
 
-// sync-free sweeps (kernels_sweep.hip); one per sweep direction
+// sync-free sweeps (kernels_sweep.hip); one per sweep direction, from the
+// host plan (SweepPlan in preprocess.h)
 struct SweepAsync {
-    const int* seq;             // swept points (device numbers) in execution order
+    const int* seq;             // visited points (device numbers) in execution order
     int count;
     const long long* take_ptr;  // [count + 1]: position t takes take[take_ptr[t] .. take_ptr[t+1])
-    const unsigned* take;       // neighbour k << 3 | direction list of every taken entry
+    const unsigned* take;       // k << 4 | wait << 3 | direction (7: supersonic copy) of every taken entry
+    double* wgt;                // [entries] least-squares weight of each entry (set at bind)
+    const int* old_pts;         // points some entry reads as old: pold is taken for them
+    int n_old;
+    const int* old_src;         // copy sources read as old: dold is taken for them
+    int n_old_src;
+    const int* wait_src;        // copy sources read as this sweep's delU: marked pending
+    int n_wait_src;
+    double* pub;                // [5n] published prim(U + delU) of the positions (completion marks)
+    double* pold;               // [5n] prim(U + delU) before the sweep, for old entries
+    double* dold;               // [5n] delU before the sweep, for copies from old sources (may be null)
     unsigned* ctr;              // [2] ticket counter, finished workgroups (zero between sweeps)
     int* stalls;                // waits that hit the spin limit
 };
@@ -86,6 +99,8 @@ void launch_flux_residual_reference(const DevGeom& g, const DevState& s, const F
 // explicit form of the sweep (delUn = -delt*flux_res, delUp = delUn)
 void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st);
 // implicit_forward_and_reverse_sweeps.f90
+// D_inv of the wall and interior points, and of the outer points when the
+// alternative driver sweeps them (p.backup)
 void launch_dinv(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                  hipStream_t st);
 // setup: the geometric part of every direction's D_inv term (12 doubles per point)
@@ -97,6 +112,8 @@ void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& 
                          const SweepAsync& bwd, int grid, hipStream_t st);
 // resident workgroup count used for the sync-free sweeps
 int sweep_async_grid(int count, bool unit_weight, bool mixed);
+// setup: the least-squares weight of every entry of a sweep's plan
+void launch_sweep_weights(const DevGeom& g, const SweepAsync& a, double power, hipStream_t st);
 // state_update.f90 (+ residual reduction into *rec; hist may be null)
 // per-cloud residuals of a batch, recorded in row *counter of inst_hist
 // (n_inst per row); the counter advances once per state update

@@ -49,36 +49,10 @@ __global__ void __launch_bounds__(kBlock) k_stencil_coef(DevGeom g, double* coef
     const int i = xcd_index();
     if (i >= g.n) return;
     const Frame<double> f = load_frame(g.frame, i);
-    const double4 pi4 = g.pos[i];
     for (int d = 0; d < 6; ++d) {
         const int cnt = g.dir_cnt[(long long)d * g.n + i];
         const int* list = g.lst + g.dir_off[(long long)d * g.n + i];
-        Geo6<double> geo = {0, 0, 0, 0, 0, 0};
-        for (int j = 0; j < cnt; ++j) {
-            const double4 pk = g.pos[list[j]];
-            const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
-            const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
-            const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
-            const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
-            const double w = wls_weight<UW>(dels, delt, deln, power);
-            const double dels_w = dels * w, delt_w = delt * w, deln_w = deln * w;
-            geo.xx = geo.xx + dels * dels_w;
-            geo.yy = geo.yy + delt * delt_w;
-            geo.zz = geo.zz + deln * deln_w;
-            geo.xy = geo.xy + dels * delt_w;
-            geo.yz = geo.yz + delt * deln_w;
-            geo.zx = geo.zx + deln * dels_w;
-        }
-        const int col = d >> 1;
-        const double det = cramer_det(geo);
-        double* c = coef + 18 * (long long)i + 3 * d;
-        if (cnt == 0) {
-            c[0] = c[1] = c[2] = 0.0;
-            continue;
-        }
-        c[0] = cramer_num(geo, col, 1.0, 0.0, 0.0) / det;
-        c[1] = cramer_num(geo, col, 0.0, 1.0, 0.0) / det;
-        c[2] = cramer_num(geo, col, 0.0, 0.0, 1.0) / det;
+        list_cofactors<UW>(g, i, f, list, cnt, d >> 1, power, coef + 18 * (long long)i + 3 * d);
     }
 }
 
@@ -324,6 +298,11 @@ struct Stencil {
     bool by_one_by_det;
 };
 
+// the list a sweep walks: the flux stencil's, except an outer point's z
+// term, which walks conn (implicit_LdelU_forward_sweep_outer.f90:346-347,
+// reached only from the alternative sweep driver)
+__device__ __forceinline__ Stencil sweep_stencil(const DevGeom& g, int cls, int d, int i);
+
 __device__ __forceinline__ Stencil dir_stencil(const DevGeom& g, int cls, int d, int i) {
     Stencil st;
     st.list = g.lst + g.dir_off[(long long)d * g.n + i];
@@ -332,6 +311,16 @@ __device__ __forceinline__ Stencil dir_stencil(const DevGeom& g, int cls, int d,
     st.axis = d >> 1;
     st.sign = (d & 1) ? -1 : 1;
     st.by_one_by_det = cls == 2;
+    return st;
+}
+
+__device__ __forceinline__ Stencil sweep_stencil(const DevGeom& g, int cls, int d, int i) {
+    Stencil st = dir_stencil(g, cls, d, i);
+    if (cls == 2 && d == ZPOS) {
+        st.list = g.conn + kConnCols * (long long)i;
+        st.count = g.nbhs[i];
+    }
+    st.by_one_by_det = false;
     return st;
 }
 
@@ -493,16 +482,33 @@ __device__ void dinv_geometry(const DevGeom& g, double power, int i, const Stenc
 }
 
 // setup: the numerator and determinant of all six direction lists of every
-// point, dgeo[12 i + 2 d] and dgeo[12 i + 2 d + 1] (a list's geometry does
-// not depend on the point's class)
+// point, dgeo[12 i + 2 d] and dgeo[12 i + 2 d + 1], over the lists the sweeps
+// walk (sweep_stencil).  Outer points, swept only by the alternative driver,
+// take ZPOS over conn (implicit_D_inv_outer.f90:306-353), and their ZNEG
+// slot holds what implicit_D_inv_outer.f90:360 divides 1 by: its local delt
+// is also the loop's tan2 projection (:73 ...), so after the conn loop it is
+// the last conn neighbour's tan2 offset -- (that offset, 1), or (0, 0) when
+// the point has no neighbours and delt keeps the time step.
 template <bool UW>
 __global__ void __launch_bounds__(kBlock) k_dinv_geometry(DevGeom g, double* dgeo, double power) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
     const Frame<double> f = load_frame(g.frame, i);
+    const int cls = g.status[i];
     for (int d = XPOS; d <= ZNEG; ++d) {
-        double temp, det;
-        dinv_geometry<UW>(g, power, i, dir_stencil(g, 0, d, i), f, temp, det);
+        double temp = 0.0, det = 0.0;
+        if (cls == 2 && d == ZNEG) {
+            const int nb = g.nbhs[i];
+            if (nb > 0) {
+                const double4 pi4 = g.pos[i];
+                const double4 pk = g.pos[g.conn[kConnCols * (long long)i + nb - 1]];
+                const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
+                temp = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+                det = 1.0;
+            }
+        } else {
+            dinv_geometry<UW>(g, power, i, sweep_stencil(g, cls, d, i), f, temp, det);
+        }
         dgeo[12 * (long long)i + 2 * d] = temp;
         dgeo[12 * (long long)i + 2 * d + 1] = det;
     }
@@ -533,7 +539,17 @@ __global__ void __launch_bounds__(kBlock) k_dinv(DevGeom g, DevState s, const in
     const double as = __builtin_sqrt(1.40 * s.prim[5 * (long long)i + 4] / s.prim[5 * (long long)i]);
     double d[6];
     double temp;
-    if (cls == 1) {
+    if (cls == 2) {
+        // implicit_D_inv_outer.f90:44-362: the negative halves enter with a
+        // plus sign (:171, :295), the sum is taken left to right (:360)
+        const double dGxp = dinv_dir(s, i, XPOS, gd, as);
+        const double dGxn = -dinv_dir(s, i, XNEG, gd, as);
+        const double dGyp = dinv_dir(s, i, YPOS, gd, as);
+        const double dGyn = -dinv_dir(s, i, YNEG, gd, as);
+        const double dGzp = dinv_dir(s, i, ZPOS, gd, as);
+        const double delt = gd[2 * ZNEG + 1] != 0.0 ? gd[2 * ZNEG] : delta_t;
+        temp = (1.0 / delt) - dGxp + dGxn - dGyp + dGyn - dGzp;
+    } else if (cls == 1) {
         for (int dd = XPOS; dd <= YNEG; ++dd) d[dd] = dinv_dir(s, i, dd, gd, as);
         d[4] = dinv_dir(s, i, ZNEG, gd, as);
         temp = (1.0 / delta_t) - 2.0 * (d[0] + d[1] + d[2] + d[3] + d[4]);
@@ -601,6 +617,18 @@ template <bool FORWARD, bool UW>
 __device__ void sweep_term(const DevGeom& g, const DevState& s, const FlowParams& fp, int i,
                            const Frame<double>& f, double out[5]) {
     double d[5];
+    if (g.status[i] == 2) {
+        // implicit_LdelU_forward_sweep_outer.f90:50-406: Go over the split
+        // x/y lists, Gzp over conn, summed without a factor
+        sweep_dir<FORWARD, UW>(g, s, fp, i, sweep_stencil(g, 2, XPOS, i), f, out);
+#pragma unroll 1
+        for (int dd = XNEG; dd <= ZPOS; ++dd) {
+            sweep_dir<FORWARD, UW>(g, s, fp, i, sweep_stencil(g, 2, dd, i), f, d);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) out[r] = out[r] + d[r];
+        }
+        return;
+    }
     if (g.status[i] == 1) {
         sweep_dir<FORWARD, UW>(g, s, fp, i, dir_stencil(g, 1, XPOS, i), f, out);
 #pragma unroll 1
@@ -623,12 +651,46 @@ __device__ void sweep_term(const DevGeom& g, const DevState& s, const FlowParams
     }
 }
 
+// nearest neighbour of a supersonic point (implicit_forward_and_reverse_
+// sweeps.f90:39-50, state_update.f90:84-94); skip_wall: the alternative
+// driver's status /= 1 filter (implicit_forward_and_reverse_sweeps_backup.f90:
+// 56, state_update_implicit.f90:72,89)
+__device__ int nearest_neighbour_dev(const DevGeom& g, int k, double start, bool skip_wall) {
+    const double4 pk = g.pos[k];
+    double min_dist = start;
+    int p = k;
+    for (int r = 0; r < g.nbhs[k]; ++r) {
+        const int nbh = g.conn[kConnCols * (long long)k + r];
+        const double4 pn = g.pos[nbh];
+        const double dx = pn.x - pk.x, dy = pn.y - pk.y, dz = pn.z - pk.z;
+        const double ds = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+        if (ds < min_dist && !(skip_wall && g.status[nbh] == 1)) {
+            min_dist = ds;
+            p = nbh;
+        }
+    }
+    return p;
+}
+
+// one level of a sweep, in place: the level schedule orders every point
+// after the points whose new values it reads and after the points that
+// read its old value, so in-place reads see what the reference's visiting
+// order sees
 template <bool FORWARD, bool UW>
 __global__ void __launch_bounds__(64) k_sweep_level(DevGeom g, DevState s, FlowParams fp,
                                                     const int* points, int begin, int end) {
     const int m = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= end) return;
     const int p = points[m];
+    const int cls = g.status[p];
+    if (cls == 5 || cls == 6) {
+        // implicit_forward_and_reverse_sweeps.f90:38-52, 69-83
+        const int q = nearest_neighbour_dev(g, p, fp.backup ? 100000.0 : 1000000.0, fp.backup != 0);
+        double* dU = FORWARD ? s.delUp : s.delUn;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) dU[5 * (long long)p + r] = dU[5 * (long long)q + r];
+        return;
+    }
     const Frame<double> f = load_frame(g.frame, p);
     double T5[5];
     sweep_term<FORWARD, UW>(g, s, fp, p, f, T5);
@@ -701,7 +763,10 @@ __device__ __forceinline__ MaxPair max_pair(MaxPair a, MaxPair b) {
 // EXPLICIT: the explicit form's sweep (k_explicit_sweep: delUn = delUp =
 // -delt*R for every point) is done here first, with the same arithmetic, so
 // delUn and delUp are written once instead of written and read back
-template <bool EXPLICIT>
+// BACKUP: state_update_implicit.f90:22-60, paired with the alternative
+// sweep driver: wall points keep U(4), nothing is copied into delUp here,
+// outer points take Ubar + delUn
+template <bool EXPLICIT, bool BACKUP = false>
 __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, FlowParams fp,
                                                          double* partial, int* partial_pos,
                                                          int* partial_pt, double* partial_max,
@@ -736,7 +801,7 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
             double U[5];
 #pragma unroll
             for (int r = 0; r < 5; ++r) U[r] = s.U[5 * (long long)k + r] + dU[r];
-            if (st == 1) U[3] = 0.0;
+            if (st == 1 && !BACKUP) U[3] = 0.0;
             res_sqr = (U[0] - temp) * (U[0] - temp);
             mp.v = res_sqr;
             mp.pos = g.respos[k];
@@ -746,15 +811,21 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 s.prim[5 * (long long)k + r] = prim[r];
-                if (!EXPLICIT) s.delUp[5 * (long long)k + r] = dU[r];
+                if (!EXPLICIT && !BACKUP) s.delUp[5 * (long long)k + r] = dU[r];
             }
         } else if (st == 2) {
             const Frame<double> f = load_frame(g.frame, k);
             double U[5], prim[5];
             conserved_vector_Ubar(g, s, fp, k, f, U);
-            const double dt = s.delt[k];
+            if (BACKUP) {
+                // state_update_implicit.f90:45-47
 #pragma unroll
-            for (int r = 0; r < 5; ++r) U[r] = U[r] - dt * s.flux_res[5 * (long long)k + r];
+                for (int r = 0; r < 5; ++r) U[r] = U[r] + s.delUn[5 * (long long)k + r];
+            } else {
+                const double dt = s.delt[k];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) U[r] = U[r] - dt * s.flux_res[5 * (long long)k + r];
+            }
             conserved_to_prim(U, f, prim);
 #pragma unroll
             for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = prim[r];
@@ -797,26 +868,14 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
 // state after every other point is updated (outlet list, then inlet list,
 // each in order, so a single thread walks them)
 __global__ void k_supersonic(DevGeom g, DevState s, const int* out_idx, int n_out, const int* in_idx,
-                             int n_in) {
+                             int n_in, int skip_wall) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int pass = 0; pass < 2; ++pass) {
         const int* idx = pass == 0 ? out_idx : in_idx;
         const int cnt = pass == 0 ? n_out : n_in;
         for (int m = 0; m < cnt; ++m) {
             const int k = idx[m];
-            const double4 pk = g.pos[k];
-            double min_dist = 100000.0;
-            int p = k;
-            for (int r = 0; r < g.nbhs[k]; ++r) {
-                const int nbh = g.conn[kConnCols * (long long)k + r];
-                const double4 pn = g.pos[nbh];
-                const double dx = pn.x - pk.x, dy = pn.y - pk.y, dz = pn.z - pk.z;
-                const double ds = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
-                if (ds < min_dist) {
-                    min_dist = ds;
-                    p = nbh;
-                }
-            }
+            const int p = nearest_neighbour_dev(g, k, 100000.0, skip_wall != 0);
             for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = s.prim[5 * (long long)p + r];
         }
     }
@@ -1057,10 +1116,9 @@ void launch_explicit_sweep(const DevGeom& g, const DevState& s, hipStream_t st) 
 }
 void launch_dinv(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                  hipStream_t st) {
-    const int* lists[2] = {tl.wall, tl.interior};
-    const int counts[2] = {tl.n_wall, tl.n_interior};
-    (void)p;
-    for (int c = 0; c < 2; ++c) {
+    const int* lists[3] = {tl.wall, tl.interior, tl.outer};
+    const int counts[3] = {tl.n_wall, tl.n_interior, p.backup ? tl.n_outer : 0};
+    for (int c = 0; c < 3; ++c) {
         if (counts[c] <= 0) continue;
         hipLaunchKernelGGL(k_dinv, dim3(grid_for(counts[c], kBlock)), dim3(kBlock), 0, st, g, s, lists[c], counts[c]);
     }
@@ -1092,9 +1150,11 @@ void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams&
             hipLaunchKernelGGL((k_sweep_level<false, false>), dim3(grid_for(e - b, 64)), dim3(64), 0, st, g, s, p,
                                sch.bwd_points, b, e);
     }
-    // implicit_forward_and_reverse_sweeps.f90:86-88
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,
-                       (const double*)s.delUn, 5LL * g.n);
+    // implicit_forward_and_reverse_sweeps.f90:86-88; the alternative driver
+    // leaves it to state_update_implicit.f90:98-100
+    if (!p.backup)
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,
+                           (const double*)s.delUn, 5LL * g.n);
 }
 int state_update_blocks(int n) { return grid_for(n, kBlock); }
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
@@ -1103,15 +1163,23 @@ void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& 
     // partial buffer layout: [sum | max] doubles, [pos | point] ints
     double* partial_max = partial + n_partial;
     int* partial_pt = partial_pos + n_partial;
+    const bool backup = p.backup && !with_explicit_sweep;
     if (with_explicit_sweep)
-        hipLaunchKernelGGL(k_state_update<true>, dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial, partial_pos,
-                           partial_pt, partial_max, ir ? ir->ressq : nullptr);
+        hipLaunchKernelGGL((k_state_update<true, false>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
+                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
+    else if (backup)
+        hipLaunchKernelGGL((k_state_update<false, true>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
+                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
     else
-        hipLaunchKernelGGL(k_state_update<false>, dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
+        hipLaunchKernelGGL((k_state_update<false, false>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
                            partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
     if (tl.n_sup_in + tl.n_sup_out > 0)
         hipLaunchKernelGGL(k_supersonic, dim3(1), dim3(1), 0, st, g, s, tl.sup_out, tl.n_sup_out, tl.sup_in,
-                           tl.n_sup_in);
+                           tl.n_sup_in, backup ? 1 : 0);
+    // state_update_implicit.f90:98-100
+    if (backup)
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,
+                           (const double*)s.delUn, 5LL * g.n);
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(1024), 0, st, (const double*)partial,
                        (const int*)partial_pos, (const int*)partial_pt, (const double*)partial_max, n_partial,
                        g.n, rec, hist, hist_index, ir ? (const int*)ir->counter : (const int*)nullptr);

@@ -5,49 +5,69 @@
 // neighbour with a smaller alias, already updated earlier in the same sweep
 // (implicit_LdelU_forward_sweep_interior.f90:51-117).  The backward sweep
 // (implicit_forward_and_reverse_sweeps.f90:58-85) mirrors it with delUn and
-// larger aliases.  The result is a Gauss-Seidel recurrence whose dependency
-// graph is fixed by the alias order.
+// larger aliases.  The alternative driver (implicit_forward_and_reverse_
+// sweeps_backup.f90:25-139) visits wall, interior, outer and supersonic
+// points in type order with the same alias tests, so a neighbour it visits
+// later contributes its value from before the sweep.
 //
-// Here a persistent grid of workgroups takes tickets from a counter: each
-// round a workgroup of kSweepWaves waves takes kSweepWaves consecutive
-// positions with one atomic, and wave w processes position base + w of the
-// execution order.  Every point a wave waits on has a smaller position, so
-// it was taken earlier (or in the same round) by a wave that is running.
-// The lowest unfinished position never waits, and the barrier between a
-// workgroup's rounds only delays its next positions, which are higher than
-// every position taken, so every wait ends.
+// The host plan (preprocess.h, SweepPlan) lists, per execution position,
+// the entries the point takes, each packed as k << 4 | wait << 3 | d:
+// direction list d (kTakeCopy for a supersonic point's copy of its nearest
+// neighbour), and whether the entry reads this sweep's value of k ("wait":
+// k executes earlier) or the value from before the sweep ("old").  Every
+// entry's least-squares weight w (1/dist**power times the Cramer cofactors of
+// its list applied to the entry's offset) is computed once, at bind
+// (k_sweep_weights).  A position's sweep term is sum_e (Gnew_e - Gold_e)*w_e,
+// which equals the reference's Cramer solve of the summed products up to
+// rounding.
 //
-// The values themselves are the completion flags.  Before a sweep, a kernel
-// marks the five outputs of every swept point with a signalling-NaN
-// pattern that no arithmetic produces (kPending).  A point writes its five
-// results once, with agent-scope relaxed atomic stores; a wave first
-// computes everything that does not depend on this sweep's values
-// (geometry, Cramer weights, the old fluxes Gold), then polls its
-// dependencies' values with agent-scope relaxed atomic loads until none is
-// pending, and uses the values so read.  Each 64-bit value is written once
-// and checked on its own, so no ordering between locations -- and no cache
-// write-back or invalidation -- is needed.  A wait that exceeds the spin
-// limit is counted in the residual record and the host reports an error;
-// the kernel still runs to completion.
+// Per sweep:
+//   k_sweep_snap   the values old entries read: prim(U_k + delU_k) of every
+//                  point an entry reads as old (pold), and delU of every copy
+//                  source read as old (dold), before anything is overwritten;
+//   k_sweep_mark   marks pub (the published primitive state) of every
+//                  position, and delU of every copy source read as this
+//                  sweep's value, with a signalling-NaN pattern that no
+//                  arithmetic produces (kPending);
+//   k_sweep_async  a persistent grid of workgroups of kSweepWaves waves; each
+//                  round thread 0 takes kSweepWaves consecutive positions
+//                  with one atomic and wave w works on position base + w.
+//                  Lane e of the wave takes entry e (e + 64, ... for longer
+//                  lists): Gold from the point's prim before the sweep, then
+//                  the neighbour's published primitive state -- polled until
+//                  it is no longer pending for a wait entry, read from pold
+//                  for an old one -- and Gnew from it.  The wave's sum is a
+//                  fixed-order DPP reduction.  Lane 0 stores the point's delU
+//                  and its published state prim(U_i + delU_i), computed once
+//                  here instead of once per entry that reads it.
 //
-// One wave of 64 lanes handles one point.  The host lists, per execution
-// position and sweep direction, the entries of the point's direction lists
-// that this sweep takes (the neighbours with a smaller alias in the forward
-// sweep, a larger one in the backward sweep; a wall point's z term walks
-// conn, so its ZPOS list is skipped), each packed as k << 3 | direction.
-// Lane e takes entry e (e + 64, ... for longer lists), and the sum over all
-// entries (the sweep term is the sum of the directions' dG, times 2 at wall
-// points) is a fixed-order DPP reduction within the wave (wave_sum).  Each entry's
-// contribution is (Gnew - Gold) * w (c0*dels + c1*delt + c2*deln) with the
-// Cramer cofactors c of the list (k_stencil_coef), which equals the
-// reference's Cramer solve on the summed products up to rounding.  T is the
-// arithmetic of the split fluxes (float in the mixed form); the weights, the
-// sums, D_inv and the new delU stay FP64.
+// Memory model.  Inside k_sweep_async the only memory written is delU and
+// pub of the positions, each 64-bit value once, with agent-scope relaxed
+// atomic stores (sc1: written through to the coherent level, no dirty line
+// in any XCD's L2); the values it polls are read with agent-scope relaxed
+// atomic loads (sc1: never served from a possibly stale L2 line).  Each
+// value is checked on its own, so no ordering between locations is needed.
+// The pending marks are written with the same agent-scope atomic stores, so
+// no XCD's L2 ever holds a dirty line of these arrays that a later
+// write-back could lay over a result.  Everything else the kernel reads is
+// not written while it runs.
+//
+// Progress.  Every wait entry reads a point that executes earlier, so it was
+// taken earlier (or in the same round) by a wave that is running.  The lowest
+// unfinished position never waits on an unfinished one, and the barrier
+// between a workgroup's rounds only delays its next positions, which are
+// higher than every position taken, so every wait ends.  A wait that
+// exceeds the spin limit is counted in the residual record and the host
+// reports an error; the kernel still runs to completion.
+//
+// T is the arithmetic of the split fluxes (float in the mixed form); the
+// weights, the sums, D_inv and the new delU stay FP64.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
 #include "device_state.h"
 #include "kernels.h"
+#include "preprocess.h"
 #include "qlskum_math.h"
 
 namespace mfcfd {
@@ -71,12 +91,10 @@ __device__ __forceinline__ void store_value(double* p, double v) {
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// read the five values of a neighbour, waiting until none is pending: a
-// neighbour swept earlier in this sweep has then written them, any other
-// holds a value from before the sweep, which is never pending.  Gives up
-// after the spin limit, or at once when another wait of this sweep already
-// gave up, so a broken schedule ends the kernel instead of stalling it point
-// by point.
+// read five values written in this sweep, waiting until none is pending.
+// Gives up after the spin limit, or at once when another wait of this sweep
+// already gave up, so a broken schedule ends the kernel instead of stalling
+// it point by point.
 __device__ __forceinline__ void wait_read(const double* v5, double out[5], int* stalls) {
     int spins = 0;
     for (;;) {
@@ -131,6 +149,95 @@ __device__ __forceinline__ double wave_sum(double v) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// the list a sweep walks for point i and direction d (SweepPlan's
+// sweep_list): the flux stencil's list, except an outer point's z term,
+// which walks conn (implicit_LdelU_forward_sweep_outer.f90:346-347)
+__device__ __forceinline__ void sweep_list_dev(const DevGeom& g, int cls, int d, int i, const int*& list,
+                                               int& count) {
+    if (cls == 2 && d == ZPOS) {
+        list = g.conn + kConnCols * (long long)i;
+        count = g.nbhs[i];
+        return;
+    }
+    list = g.lst + g.dir_off[(long long)d * g.n + i];
+    count = g.dir_cnt[(long long)d * g.n + i];
+}
+
+// setup: the weight of every taken entry, one thread per position.  Entries
+// come grouped by direction; each direction's Cramer cofactors are those of
+// the whole list the reference walks (list_cofactors, the same arithmetic as
+// the flux stencils' weights).
+template <bool UW>
+__global__ void __launch_bounds__(kBlock) k_sweep_weights(DevGeom g, SweepAsync a, double power) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= a.count) return;
+    const int i = a.seq[u];
+    const int cls = g.status[i];
+    const Frame<double> f = load_frame(g.frame, i);
+    const double4 pi4 = g.pos[i];
+    double c[3] = {0.0, 0.0, 0.0};
+    int cur = -1;
+    for (long long e = a.take_ptr[u]; e < a.take_ptr[u + 1]; ++e) {
+        const unsigned packed = a.take[e];
+        const int d = (int)(packed & 7u);
+        if (d == (int)kTakeCopy) {
+            a.wgt[e] = 0.0;
+            continue;
+        }
+        if (d != cur) {
+            const int* list;
+            int count;
+            sweep_list_dev(g, cls, d, i, list, count);
+            list_cofactors<UW>(g, i, f, list, count, d >> 1, power, c);
+            cur = d;
+        }
+        const double4 pk = g.pos[packed >> kTakeShift];
+        const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
+        const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+        const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+        const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+        a.wgt[e] = wls_weight<UW>(dels, delt, deln, power) * (c[0] * dels + c[1] * delt + c[2] * deln);
+    }
+}
+
+// the values old entries read, taken before the sweep overwrites anything:
+// pold = prim(U + delU) (get_primitive_vector.f90:10-41) of the points
+// regular entries read as old, dold = delU of the copy sources read as old
+__global__ void __launch_bounds__(kBlock) k_sweep_snap(DevGeom g, DevState s, const double* dU, SweepAsync a) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < a.n_old) {
+        const int k = a.old_pts[e];
+        double U[5], prim[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) U[r] = dU[5 * (long long)k + r] + s.U[5 * (long long)k + r];
+        conserved_to_prim(U, load_frame(g.frame, k), prim);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) a.pold[5 * (long long)k + r] = prim[r];
+    } else if (e < (long long)a.n_old + a.n_old_src) {
+        const int q = a.old_src[e - a.n_old];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) a.dold[5 * (long long)q + r] = dU[5 * (long long)q + r];
+    }
+}
+
+// pending marks: pub of every position, delU of every copy source read as
+// this sweep's value; agent-scope atomic stores (no dirty L2 line)
+__global__ void __launch_bounds__(kBlock) k_sweep_mark(double* dU, SweepAsync a) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n1 = 5LL * a.count;
+    unsigned long long* dst;
+    if (e < n1) {
+        const long long t = e / 5;
+        dst = reinterpret_cast<unsigned long long*>(a.pub) + 5LL * a.seq[t] + (e - 5 * t);
+    } else if (e < n1 + 5LL * a.n_wait_src) {
+        const long long t = (e - n1) / 5;
+        dst = reinterpret_cast<unsigned long long*>(dU) + 5LL * a.wait_src[t] + (e - n1 - 5 * t);
+    } else {
+        return;
+    }
+    __hip_atomic_store(dst, kPending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <bool FORWARD, bool UW, typename T>
 __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState s, FlowParams fp,
                                                              SweepAsync a) {
@@ -140,7 +247,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
     __shared__ unsigned sh_base[2];
     const int lane = threadIdx.x & 63;
     const unsigned wave = threadIdx.x >> 6;
-    const double* dU = FORWARD ? s.delUp : s.delUn;
+    double* dU = FORWARD ? s.delUp : s.delUn;
     for (unsigned round = 0;; ++round) {
         if (threadIdx.x == 0)
             sh_base[round & 1] = __hip_atomic_fetch_add(a.ctr, (unsigned)kSweepWaves, __ATOMIC_RELAXED,
@@ -151,64 +258,79 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         const unsigned t = base + wave;
         if (t >= (unsigned)a.count) continue;   // the last, partial round
         const int i = a.seq[t];
-        const int cls = g.status[i];
-        const Frame<double> f = load_frame(g.frame, i);
-        const Frame<T> ft = frame_as<T>(f);
-        const double4 pi4 = g.pos[i];
-        // the point's own inputs of the final update, read before the wait
-        double D_inv = 0.0, own[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (lane == 0) {
-            D_inv = s.dinv[i];
-#pragma unroll
-            for (int r = 0; r < 5; ++r) own[r] = FORWARD ? s.flux_res[5 * (long long)i + r] : s.delUp[5 * (long long)i + r];
-        }
-        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         const long long tb = a.take_ptr[t], te = a.take_ptr[t + 1];
-        for (long long e = tb + lane; e < te; e += 64) {
-            const unsigned packed = a.take[e];
-            const int k = (int)(packed >> 3);
-            const int d = (int)(packed & 7u);
-            // everything that does not depend on this sweep's values first
-            const double4 pk = g.pos[k];
-            const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
-            const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
-            const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
-            const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
-            const double* c = g.coef + 18 * (long long)i + 3 * d;
-            const double wgt = wls_weight<UW>(dels, delt, deln, fp.power) * (c[0] * dels + c[1] * delt + c[2] * deln);
-            T prim_old[5], Gold[5];
+        const Frame<double> f = load_frame(g.frame, i);
+        double res[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        const unsigned first = te > tb ? a.take[tb] : 0u;
+        if (te > tb && (first & 7u) == kTakeCopy) {
+            // supersonic point: the nearest neighbour's delU
+            // (implicit_forward_and_reverse_sweeps.f90:38-52, 69-83)
+            if (lane == 0) {
+                const long long q = (long long)(first >> kTakeShift);
+                if (first & kTakeWait) {
+                    wait_read(dU + 5 * q, res, a.stalls);
+                } else {
 #pragma unroll
-            for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * (long long)k + r]);
-            flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
-            double Uk[5];
-#pragma unroll
-            for (int r = 0; r < 5; ++r) Uk[r] = s.U[5 * (long long)k + r];
-            const Frame<double> fk = load_frame(g.frame, k);
-            // then this neighbour's values of the current sweep
-            double dk[5], U[5], prim[5];
-            wait_read(dU + 5 * (long long)k, dk, a.stalls);
-#pragma unroll
-            for (int r = 0; r < 5; ++r) U[r] = dk[r] + Uk[r];
-            conserved_to_prim(U, fk, prim);
-            T pt[5], Gnew[5];
-#pragma unroll
-            for (int r = 0; r < 5; ++r) pt[r] = T(prim[r]);
-            flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
-#pragma unroll
-            for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
-        }
-#pragma unroll
-        for (int r = 0; r < 5; ++r) acc[r] = wave_sum(acc[r]);
-        if (lane == 0) {
-            if (FORWARD) {
-#pragma unroll
-                for (int r = 0; r < 5; ++r)
-                    store_value(s.delUp + 5 * (long long)i + r, -(D_inv * (own[r] + (cls == 1 ? 2.0 * acc[r] : acc[r]))));
-            } else {
-#pragma unroll
-                for (int r = 0; r < 5; ++r)
-                    store_value(s.delUn + 5 * (long long)i + r, own[r] - D_inv * (cls == 1 ? 2.0 * acc[r] : acc[r]));
+                    for (int r = 0; r < 5; ++r) res[r] = a.dold[5 * q + r];
+                }
             }
+        } else {
+            const int cls = g.status[i];
+            const Frame<T> ft = frame_as<T>(f);
+            // the point's own inputs of the final update, read before the wait
+            double D_inv = 0.0, own[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            if (lane == 0) {
+                D_inv = s.dinv[i];
+#pragma unroll
+                for (int r = 0; r < 5; ++r)
+                    own[r] = FORWARD ? s.flux_res[5 * (long long)i + r] : s.delUp[5 * (long long)i + r];
+            }
+            double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            for (long long e = tb + lane; e < te; e += 64) {
+                const unsigned packed = a.take[e];
+                const long long k = (long long)(packed >> kTakeShift);
+                const int d = (int)(packed & 7u);
+                const double wgt = a.wgt[e];
+                // the old flux first: it does not depend on this sweep
+                T prim_old[5], Gold[5];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k + r]);
+                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
+                double pk[5];
+                if (packed & kTakeWait) {
+                    wait_read(a.pub + 5 * k, pk, a.stalls);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 5; ++r) pk[r] = a.pold[5 * k + r];
+                }
+                T pt[5], Gnew[5];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) pt[r] = T(pk[r]);
+                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
+#pragma unroll
+                for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
+            }
+#pragma unroll
+            for (int r = 0; r < 5; ++r) acc[r] = wave_sum(acc[r]);
+            if (lane == 0) {
+                // wall terms are doubled (implicit_LdelU_forward_sweep_wall.f90:405)
+#pragma unroll
+                for (int r = 0; r < 5; ++r) {
+                    const double term = cls == 1 ? 2.0 * acc[r] : acc[r];
+                    res[r] = FORWARD ? -(D_inv * (own[r] + term)) : own[r] - D_inv * term;
+                }
+            }
+        }
+        if (lane == 0) {
+            double U[5], pp[5];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                store_value(dU + 5 * (long long)i + r, res[r]);
+                U[r] = res[r] + s.U[5 * (long long)i + r];
+            }
+            conserved_to_prim(U, f, pp);
+#pragma unroll
+            for (int r = 0; r < 5; ++r) store_value(a.pub + 5 * (long long)i + r, pp[r]);
         }
     }
     // the last workgroup out resets the counters for the next sweep
@@ -219,14 +341,6 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
             __hip_atomic_store(a.ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-}
-
-// mark the outputs of the swept points as not yet written
-__global__ void k_mark_pending(double* out, const int* seq, int count) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= 5LL * count) return;
-    const long long t = e / 5;
-    reinterpret_cast<unsigned long long*>(out)[5LL * seq[t] + (e - 5 * t)] = kPending;
 }
 
 __global__ void k_copy_sweep(double* dst, const double* src, long long n) {
@@ -254,14 +368,24 @@ int pair_grid(int count) {
     return a < b ? a : b;
 }
 
+template <bool FORWARD, bool UW, typename T>
+void launch_one(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& a, int grid,
+                hipStream_t st) {
+    double* dU = FORWARD ? s.delUp : s.delUn;
+    const long long n_snap = (long long)a.n_old + a.n_old_src;
+    if (n_snap > 0)
+        hipLaunchKernelGGL(k_sweep_snap, dim3(grid_for(n_snap, kBlock)), dim3(kBlock), 0, st, g, s,
+                           (const double*)dU, a);
+    const long long n_mark = 5LL * ((long long)a.count + a.n_wait_src);
+    hipLaunchKernelGGL(k_sweep_mark, dim3(grid_for(n_mark, kBlock)), dim3(kBlock), 0, st, dU, a);
+    hipLaunchKernelGGL((k_sweep_async<FORWARD, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, a);
+}
+
 template <bool UW, typename T>
 void launch_pair(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
                  const SweepAsync& bwd, int grid, hipStream_t st) {
-    const dim3 mg(grid_for(5LL * fwd.count, kBlock));
-    hipLaunchKernelGGL(k_mark_pending, mg, dim3(kBlock), 0, st, s.delUp, fwd.seq, fwd.count);
-    hipLaunchKernelGGL((k_sweep_async<true, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, fwd);
-    hipLaunchKernelGGL(k_mark_pending, mg, dim3(kBlock), 0, st, s.delUn, bwd.seq, bwd.count);
-    hipLaunchKernelGGL((k_sweep_async<false, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, bwd);
+    launch_one<true, UW, T>(g, s, p, fwd, grid, st);
+    launch_one<false, UW, T>(g, s, p, bwd, grid, st);
 }
 
 }  // namespace
@@ -270,6 +394,14 @@ int sweep_async_grid(int count, bool unit_weight, bool mixed) {
     if (count <= 0) return 0;
     if (mixed) return unit_weight ? pair_grid<true, float>(count) : pair_grid<false, float>(count);
     return unit_weight ? pair_grid<true, double>(count) : pair_grid<false, double>(count);
+}
+
+void launch_sweep_weights(const DevGeom& g, const SweepAsync& a, double power, hipStream_t st) {
+    if (a.count <= 0) return;
+    if (power == 0.0)
+        hipLaunchKernelGGL(k_sweep_weights<true>, dim3(grid_for(a.count, kBlock)), dim3(kBlock), 0, st, g, a, power);
+    else
+        hipLaunchKernelGGL(k_sweep_weights<false>, dim3(grid_for(a.count, kBlock)), dim3(kBlock), 0, st, g, a, power);
 }
 
 void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
@@ -287,9 +419,11 @@ void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& 
                 launch_pair<false, double>(g, s, p, fwd, bwd, grid, st);
         }
     }
-    // implicit_forward_and_reverse_sweeps.f90:86-88
-    hipLaunchKernelGGL(k_copy_sweep, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,
-                       (const double*)s.delUn, 5LL * g.n);
+    // implicit_forward_and_reverse_sweeps.f90:86-88; the alternative driver
+    // leaves it to state_update_implicit.f90:98-100
+    if (!p.backup)
+        hipLaunchKernelGGL(k_copy_sweep, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,
+                           (const double*)s.delUn, 5LL * g.n);
 }
 
 }  // namespace mfcfd

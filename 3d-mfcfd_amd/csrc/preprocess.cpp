@@ -2,8 +2,9 @@
 //
 //  * split stencils   generate_connectivity.f90:12-246
 //  * alias order      implicit_aliasing.f90:13-44
-//  * sweep schedule   dependency levels of the alias-ordered Gauss-Seidel
-//                     sweeps of implicit_forward_and_reverse_sweeps.f90:26-84
+//  * sweep plans      execution order, dependency levels and taken entries
+//                     of the LU-SGS sweeps (implicit_forward_and_reverse_
+//                     sweeps.f90:26-85 and its alternative, ..._backup.f90)
 //
 // The split lists are produced in the reference's (22,N) column-major
 // storage: a list that grows past 22 entries keeps writing into the next
@@ -12,6 +13,7 @@
 #include "preprocess.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace mfcfd {
@@ -90,75 +92,142 @@ void aliasing(int n, const int* nbhs, const int* conn, int* alias, int* pwa) {
     }
 }
 
-// Dependencies of swept point p (0-based) in one sweep: every entry of the
-// lists its LdelU/UdelU visits (x/y split lists, interior z split lists or
-// the full conn for wall z) whose alias is below (forward) or above
-// (backward) alias(p) and which is itself swept.
-static void for_each_dep(const Schedule::Inputs& in, int p, bool forward, std::vector<int>& out,
-                         std::vector<int>& mark, int stamp) {
-    out.clear();
-    const int ap = in.alias[p];
-    auto consider = [&](int k) {
-        if (k < 0 || k >= in.n) return;
-        if (!in.swept[k]) return;
-        const int ak = in.alias[k];
-        if (forward ? !(ak < ap) : !(ak > ap)) return;
-        if (mark[k] == stamp) return;
-        mark[k] = stamp;
-        out.push_back(k);
-    };
-    const int dirs = in.status[p] == 0 ? 6 : 4;
-    for (int d = 0; d < dirs; ++d) {
-        const int c = in.split_cnt[d][p];
-        for (int j = 0; j < c; ++j) consider(in.split[d][(long long)p * 22 + j] - 1);
+void sweep_list(const SweepPlanInputs& in, int i, int d, const int*& list, int& count) {
+    const int st = in.status[i];
+    list = nullptr;
+    count = 0;
+    if (st == 1 && d == 4) return;  // ZPOS: the wall's z term walks conn as ZNEG
+    if (st == 2 && d == 5) return;
+    if (st == 2 && d == 4) {
+        list = in.conn + 27LL * i;
+        count = in.nbhs[i];
+        return;
     }
-    if (in.status[p] == 1)
-        for (int j = 0; j < in.nbhs[p]; ++j) consider(in.conn[27LL * p + j] - 1);
+    list = in.lst + in.off[(long long)d * in.n + i];
+    count = in.cnt[(long long)d * in.n + i];
 }
 
-void Schedule::build(const Inputs& in) {
-    const int n = in.n;
-    order.clear();
-    for (int i = 0; i < n; ++i) {
-        const int p = in.pwa[i] - 1;
-        if (p < 0) continue;
-        if (in.status[p] == 0 || in.status[p] == 1) order.push_back(p);
+int nearest_neighbour(const SweepPlanInputs& in, int p, double start, bool skip_wall) {
+    double min_dist = start;
+    int q = p;
+    for (int r = 0; r < in.nbhs[p]; ++r) {
+        const int nbh = in.conn[27LL * p + r];
+        const double dx = in.x[nbh] - in.x[p];
+        const double dy = in.y[nbh] - in.y[p];
+        const double dz = in.z[nbh] - in.z[p];
+        const double ds = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (ds < min_dist && !(skip_wall && in.status[nbh] == 1)) {
+            min_dist = ds;
+            q = nbh;
+        }
     }
-    std::vector<int> lev(n, 0), mark(n, -1), deps;
-    auto levels_for = [&](bool forward, std::vector<int>& ptr, std::vector<int>& pts, int& nlev,
-                          std::vector<long long>& dptr, std::vector<int>& dep) {
-        std::fill(lev.begin(), lev.end(), 0);
-        std::fill(mark.begin(), mark.end(), -1);
-        int maxl = 0;
-        const int m = (int)order.size();
-        dptr.assign(m + 1, 0);
-        dep.clear();
-        for (int t = 0; t < m; ++t) {
-            const int p = forward ? order[t] : order[m - 1 - t];
-            for_each_dep(in, p, forward, deps, mark, p);
-            dep.insert(dep.end(), deps.begin(), deps.end());
-            dptr[t + 1] = (long long)dep.size();
-            int l = 0;
-            for (int k : deps) l = std::max(l, lev[k]);
-            lev[p] = l + 1;
-            maxl = std::max(maxl, l + 1);
+    return q;
+}
+
+namespace {
+
+int build_dir(const SweepPlanInputs& in, bool forward, SweepDirPlan& out) {
+    const int n = in.n;
+    const std::vector<int>& visit = forward ? in.visit_fwd : in.visit_bwd;
+    const int m = (int)visit.size();
+    std::vector<int> at(n, -1);
+    for (int u = 0; u < m; ++u) at[visit[u]] = u;
+    // entries in visiting order, then levels
+    std::vector<long long> vptr(m + 1, 0);
+    std::vector<unsigned> vtake;
+    vtake.reserve((size_t)m * 40);
+    std::vector<int> level(n, 0);
+    std::vector<std::vector<int>> war(n);   // earlier points that read k as old
+    std::vector<char> is_old(n, 0), is_old_src(n, 0), is_wait_src(n, 0);
+    const double start = in.backup ? 100000.0 : 1000000.0;
+    int max_level = 0;
+    for (int u = 0; u < m; ++u) {
+        const int i = visit[u];
+        const int st = in.status[i];
+        int lev = 0;
+        if (st == 5 || st == 6) {
+            const int q = nearest_neighbour(in, i, start, in.backup);
+            const bool wait = at[q] >= 0 && at[q] < u;
+            vtake.push_back(((unsigned)q << kTakeShift) | (wait ? kTakeWait : 0u) | kTakeCopy);
+            if (wait) {
+                lev = std::max(lev, level[q]);
+                is_wait_src[q] = 1;
+            } else {
+                is_old_src[q] = 1;
+                if (at[q] > u) war[q].push_back(i);
+            }
+        } else {
+            const int ai = in.alias[i];
+            for (int d = 0; d < 6; ++d) {
+                const int* list;
+                int count;
+                sweep_list(in, i, d, list, count);
+                for (int j = 0; j < count; ++j) {
+                    const int k = list[j];
+                    const int ak = in.alias[k];
+                    if (forward ? !(ak < ai) : !(ak > ai)) continue;
+                    const bool wait = at[k] >= 0 && at[k] < u;
+                    vtake.push_back(((unsigned)k << kTakeShift) | (wait ? kTakeWait : 0u) | (unsigned)d);
+                    if (wait) {
+                        lev = std::max(lev, level[k]);
+                    } else {
+                        is_old[k] = 1;
+                        if (at[k] > u) war[k].push_back(i);
+                    }
+                }
+            }
         }
-        nlev = maxl;
-        // level l (0-based) covers pts[ptr[l] .. ptr[l+1]); sweep order is
-        // kept inside a level so the schedule is deterministic
-        std::vector<int> counts(maxl + 1, 0);
-        for (int p : order) counts[lev[p]]++;
-        ptr.assign(maxl + 1, 0);
-        for (int l = 0; l < maxl; ++l) ptr[l + 1] = ptr[l] + counts[l + 1];
-        pts.assign(order.size(), 0);
-        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-        for (int t = 0; t < m; ++t) {
-            const int p = forward ? order[t] : order[m - 1 - t];
-            pts[fill[lev[p] - 1]++] = p;
-        }
-    };
-    levels_for(true, fwd_ptr, fwd_points, n_levels_fwd, fwd_dep_ptr, fwd_dep);
-    levels_for(false, bwd_ptr, bwd_points, n_levels_bwd, bwd_dep_ptr, bwd_dep);
+        for (int j : war[i]) lev = std::max(lev, level[j]);
+        level[i] = lev + 1;
+        max_level = std::max(max_level, lev + 1);
+        vptr[u + 1] = (long long)vtake.size();
+    }
+    // execution order: by level, visiting order inside a level
+    std::vector<int> counts(max_level + 2, 0);
+    for (int u = 0; u < m; ++u) counts[level[visit[u]]]++;
+    out.level_ptr.assign(max_level + 1, 0);
+    for (int l = 0; l < max_level; ++l) out.level_ptr[l + 1] = out.level_ptr[l] + counts[l + 1];
+    std::vector<int> fill(out.level_ptr.begin(), out.level_ptr.end() - 1);
+    std::vector<int> order(m);
+    for (int u = 0; u < m; ++u) order[fill[level[visit[u]] - 1]++] = u;
+    out.seq.resize(m);
+    out.take_ptr.assign(m + 1, 0);
+    out.take.clear();
+    out.take.reserve(vtake.size());
+    std::vector<int> exec_at(n, -1);
+    for (int e = 0; e < m; ++e) {
+        const int u = order[e];
+        out.seq[e] = visit[u];
+        exec_at[visit[u]] = e;
+        out.take.insert(out.take.end(), vtake.begin() + vptr[u], vtake.begin() + vptr[u + 1]);
+        out.take_ptr[e + 1] = (long long)out.take.size();
+    }
+    // every wait entry must read a point that executes earlier
+    for (int e = 0; e < m; ++e)
+        for (long long t = out.take_ptr[e]; t < out.take_ptr[e + 1]; ++t)
+            if (out.take[t] & kTakeWait) {
+                const int k = (int)(out.take[t] >> kTakeShift);
+                if (exec_at[k] < 0 || exec_at[k] >= e) return -1;
+            }
+    out.old_pts.clear();
+    out.old_src.clear();
+    out.wait_src.clear();
+    for (int k = 0; k < n; ++k) {
+        if (is_old[k]) out.old_pts.push_back(k);
+        if (is_old_src[k]) out.old_src.push_back(k);
+        if (is_wait_src[k]) out.wait_src.push_back(k);
+    }
+    return 0;
+}
+
+}  // namespace
+
+int SweepPlan::build(const SweepPlanInputs& in) {
+    backup = in.backup;
+    n_visited = (int)in.visit_fwd.size();
+    if (build_dir(in, true, fwd)) return -1;
+    if (build_dir(in, false, bwd)) return -1;
+    return 0;
 }
 
 }  // namespace mfcfd

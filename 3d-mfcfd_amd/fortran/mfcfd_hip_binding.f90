@@ -12,7 +12,7 @@ module mfcfd_hip_binding
     use iso_c_binding
     implicit none
 
-    integer(c_int), parameter :: MFCFD_EXPLICIT = 0, MFCFD_IMPLICIT = 1
+    integer(c_int), parameter :: MFCFD_EXPLICIT = 0, MFCFD_IMPLICIT = 1, MFCFD_IMPLICIT_BACKUP = 2
     integer(c_int), parameter :: MFCFD_FP64 = 0, MFCFD_FLUX_FUSED = 0, MFCFD_SWEEP_ASYNC = 0
     integer(c_int), parameter :: ARR_PRIM = 0, ARR_U = 1, ARR_Q = 2, ARR_QM = 3, ARR_DQ = 4, &
                                  ARR_DELT = 5, ARR_FLUX_RES = 6, ARR_DELUP = 7, ARR_DELUN = 8
@@ -167,6 +167,7 @@ contains
         p%mode = MFCFD_IMPLICIT
         call get_environment_variable('MFCFD_ITERATION_FORM', form, status=stat)
         if (stat == 0 .and. trim(form) == 'explicit') p%mode = MFCFD_EXPLICIT
+        if (stat == 0 .and. trim(form) == 'implicit_backup') p%mode = MFCFD_IMPLICIT_BACKUP
         p%precision = MFCFD_FP64
         p%max_iters = max_iters
         p%flux_order = MFCFD_FLUX_FUSED

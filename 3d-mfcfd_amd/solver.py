@@ -21,7 +21,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmfcfd_hip.so")
 
-EXPLICIT, IMPLICIT = 0, 1
+EXPLICIT, IMPLICIT, IMPLICIT_BACKUP = 0, 1, 2
+# iteration forms by name: the explicit form, the canonical implicit pairing
+# (implicit_forward_and_reverse_sweeps.f90 + state_update.f90) and the
+# alternative one (implicit_forward_and_reverse_sweeps_backup.f90 +
+# state_update_implicit.f90)
+MODES = {"explicit": EXPLICIT, "implicit": IMPLICIT, "implicit_backup": IMPLICIT_BACKUP}
 FP64, MIXED = 0, 1
 ARR = {"prim": 0, "U": 1, "q": 2, "qm": 3, "dq": 4, "delt": 5, "flux_res": 6, "delUp": 7,
        "delUn": 8}
@@ -224,7 +229,7 @@ class Solver:
         self._ctx = C.c_void_p()
         _check(L.mfcfd_create(device, C.byref(self._ctx)))
         self.params = Params(mach, aoa, cfl, vl_const, power, inner_iterations,
-                             IMPLICIT if mode == "implicit" else EXPLICIT,
+                             MODES[mode],
                              MIXED if precision == "mixed" else FP64, max_iters,
                              1 if flux_order == "lists" else 0,
                              1 if sweep_schedule == "levels" else 0)

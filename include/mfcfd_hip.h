@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MFCFD_ABI_VERSION 4
+#define MFCFD_ABI_VERSION 5
 
 /* error codes */
 #define MFCFD_OK 0
@@ -36,7 +36,13 @@ extern "C" {
 
 /* iteration forms */
 #define MFCFD_EXPLICIT 0       /* U <- U - delt*R (state_update.f90:35,67) */
-#define MFCFD_IMPLICIT 1       /* LU-SGS sweeps (implicit_forward_and_reverse_sweeps.f90) */
+#define MFCFD_IMPLICIT 1       /* LU-SGS sweeps (implicit_forward_and_reverse_sweeps.f90
+                                  with state_update.f90) */
+/* the reference's alternative implicit pairing: the type-ordered sweeps that
+ * also sweep outer points (implicit_forward_and_reverse_sweeps_backup.f90:
+ * 16-141, with implicit_D_inv_outer.f90 and the *_outer sweep terms) with
+ * state_update_implicit.f90:14-102 (outer Ubar + delUn, no wall U(4) = 0) */
+#define MFCFD_IMPLICIT_BACKUP 2
 #define MFCFD_FP64 0
 /* MIXED: q, qm, dq, the q-derivative right-hand sides and the split fluxes
  * (flux residual and LU-SGS sweeps) in FP32; positions, least-squares
@@ -67,7 +73,7 @@ typedef struct mfcfd_params {
     double vl_const;         /* VL_CONST (Venkatakrishnan constant) */
     double power;            /* power (least-squares weight exponent) */
     int inner_iterations;    /* inner_iterations (second-order q-derivatives) */
-    int mode;                /* MFCFD_EXPLICIT or MFCFD_IMPLICIT */
+    int mode;                /* MFCFD_EXPLICIT, MFCFD_IMPLICIT or MFCFD_IMPLICIT_BACKUP */
     int precision;           /* MFCFD_FP64 or MFCFD_MIXED */
     int max_iters;           /* max_iters (used by the Fortran drop-in's final download) */
     int flux_order;          /* MFCFD_FLUX_FUSED (default) or MFCFD_FLUX_LISTS */

@@ -60,6 +60,15 @@ struct DevBuf {
         if (count == 0) return hipSuccess;
         return hipMalloc((void**)&p, count * sizeof(T));
     }
+    // fine-grained device memory: agent-scope atomic loads and stores to it
+    // are coherent across the XCDs while a kernel runs (hipMalloc's
+    // coarse-grained memory is coherent only at kernel boundaries)
+    hipError_t alloc_fine(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipExtMallocWithFlags((void**)&p, count * sizeof(T), hipDeviceMallocFinegrained);
+    }
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -170,7 +179,11 @@ struct mfcfd_ctx {
     };
     DevSweepDir sw_fwd, sw_bwd;
     int plan_form = -1;    // iteration form the plan was built for (-1: none)
-    DevBuf<double> pub, pold, dold;   // sweep values: published, before the sweep (prim / delU)
+    // sweep values: published prim (fine-grained: polled while the sweep
+    // runs), prim and delU before the sweep, published delU of the copy
+    // sources (fine-grained) and each point's slot in it (-1: none)
+    DevBuf<double> pub, pold, dold, dsrc;
+    DevBuf<int> src_slot_f, src_slot_b;
     DevBuf<unsigned> sweep_ctr;
     // host inputs of plan building (device order), kept from bind
     std::vector<int> h_status, h_alias, h_nbhs, h_conn, h_cnt, h_lst;
@@ -202,7 +215,7 @@ struct mfcfd_ctx {
     int n_fused_interior_pts = 0, n_fused_general_pts = 0;
     std::vector<int> host_class;   // per host point: 0 fused interior, 1 fused general, 2 list-by-list
     // HIP events around each fused interior flux launch (mfcfd_kernel_timing)
-    bool time_flux = false;
+    int time_kernel = 0;   // MFCFD_TIME_* of mfcfd_kernel_timing (0: off)
     std::vector<hipEvent_t> tev;
     size_t tev_used = 0;
     double tev_ms = 0.0;
@@ -263,6 +276,9 @@ struct mfcfd_ctx {
         pub.release();
         pold.release();
         dold.release();
+        dsrc.release();
+        src_slot_f.release();
+        src_slot_b.release();
         q32.release();
         qm32.release();
         dq32.release();
@@ -338,13 +354,32 @@ int launch_error(const char* what) {
     return MFCFD_OK;
 }
 
+// index into c->tev of a pair of timing events (begin, end) for one more
+// timed launch, or -1.  An index, not a pointer: a later call may grow (and
+// move) the vector.
+long long timing_pair(mfcfd_ctx* c) {
+    if (c->tev_used + 2 > c->tev.size()) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess) return -1;
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            return -1;
+        }
+        c->tev.push_back(a);
+        c->tev.push_back(b);
+    }
+    const long long at = (long long)c->tev_used;
+    c->tev_used += 2;
+    return at;
+}
+
 void flux(mfcfd_ctx* c, int parts) {
     if (c->params.flux_order == MFCFD_FLUX_LISTS) {
         launch_flux_residual_reference(c->geom(), c->state(), c->fp, c->tl, c->stream);
         return;
     }
     const LaunchShapes& ls = c->shapes;
-    if (c->time_flux && (parts & 1)) {
+    if (c->time_kernel == MFCFD_TIME_FLUX_INTERIOR && (parts & 1)) {
         if (c->tev_used + 2 > c->tev.size()) {
             hipEvent_t a, b;
             if (hipEventCreate(&a) == hipSuccess) {
@@ -398,8 +433,11 @@ SweepAsync sweep_args(mfcfd_ctx* c, mfcfd_ctx::DevSweepDir& d) {
     a.pub = c->pub.p;
     a.pold = c->pold.p;
     a.dold = c->dold.p;
+    a.dsrc = c->dsrc.p;
+    a.src_slot = (&d == &c->sw_fwd) ? c->src_slot_f.p : c->src_slot_b.p;
     a.ctr = c->sweep_ctr.p;
     a.stalls = &c->rec.p->stalls;
+    a.zero = 0;
     return a;
 }
 
@@ -429,7 +467,24 @@ int do_sweeps(mfcfd_ctx* c) {
     const SweepAsync fwd = sweep_args(c, c->sw_fwd);
     const SweepAsync bwd = sweep_args(c, c->sw_bwd);
     const int grid = c->fp.mixed ? c->sweep_grid_mixed : c->sweep_grid;
-    launch_sweeps_async(g, s, c->fp, fwd, bwd, std::max(1, grid / std::max(1, c->shapes.sweep_div)), c->stream);
+    // events around each sweep kernel launch (mfcfd_kernel_timing)
+    hipEvent_t ev[4];
+    hipEvent_t* evp = nullptr;
+    if (c->time_kernel == MFCFD_TIME_SWEEPS) {
+        const long long a = timing_pair(c);
+        const long long b = a >= 0 ? timing_pair(c) : -1;
+        if (a >= 0 && b >= 0) {
+            for (int e = 0; e < 2; ++e) {
+                ev[e] = c->tev[a + e];
+                ev[2 + e] = c->tev[b + e];
+            }
+            evp = ev;
+        } else if (a >= 0) {
+            c->tev_used -= 2;
+        }
+    }
+    launch_sweeps_async(g, s, c->fp, fwd, bwd, std::max(1, grid / std::max(1, c->shapes.sweep_div)), c->stream,
+                        evp);
     return launch_error("implicit sweeps");
 }
 
@@ -498,7 +553,21 @@ int build_plan(mfcfd_ctx* c, int form) {
     };
     if ((rc = put(c->sw_fwd, plan.fwd))) return rc;
     if ((rc = put(c->sw_bwd, plan.bwd))) return rc;
-    if (c->pub.n == 0 && c->pub.alloc(5ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep values");
+    if (c->pub.n == 0 && c->pub.alloc_fine(5ULL * n) != hipSuccess)
+        return fail(MFCFD_ENOMEM, "sweep values (fine-grained)");
+    // copy sources read as this sweep's delU: a slot each in dsrc
+    const size_t n_src = std::max(plan.fwd.wait_src.size(), plan.bwd.wait_src.size());
+    if (n_src > 0) {
+        if (c->dsrc.n < 5 * n_src && c->dsrc.alloc_fine(5 * n_src) != hipSuccess)
+            return fail(MFCFD_ENOMEM, "sweep values (fine-grained)");
+        for (int dir = 0; dir < 2; ++dir) {
+            const std::vector<int>& ws = dir == 0 ? plan.fwd.wait_src : plan.bwd.wait_src;
+            std::vector<int> slot(n, -1);
+            for (size_t j = 0; j < ws.size(); ++j) slot[ws[j]] = (int)j;
+            if ((rc = upload(dir == 0 ? c->src_slot_f : c->src_slot_b, slot.data(), slot.size(), st))) return rc;
+            HIP_TRY(hipStreamSynchronize(st));   // slot ends here
+        }
+    }
     if (c->pold.n == 0 && c->pold.alloc(5ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep values");
     const bool copies = !plan.fwd.old_src.empty() || !plan.bwd.old_src.empty();
     if (copies && c->dold.n == 0 && c->dold.alloc(5ULL * n) != hipSuccess)
@@ -1327,7 +1396,7 @@ int mfcfd_run_graph(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, dou
     if (rc) return rc;
     if (n_iters < 0) return fail(MFCFD_EINVAL, "n_iters < 0");
     if (n_iters == 0) return MFCFD_OK;
-    if (c->time_flux) return fail(MFCFD_ESTATE, "kernel timing records events; disable it before graph runs");
+    if (c->time_kernel) return fail(MFCFD_ESTATE, "kernel timing records events; disable it before graph runs");
     // plan building allocates and copies: never inside the capture
     if ((rc = prepare_plan(c))) return rc;
     // the history buffers' addresses are captured: grow them before capture
@@ -1697,7 +1766,8 @@ int mfcfd_point_class_map(mfcfd_ctx* c, int* out) {
 int mfcfd_kernel_timing(mfcfd_ctx* c, int enable) {
     if (!c) return fail(MFCFD_EINVAL, "null context");
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->time_flux = enable != 0;
+    if (enable < 0 || enable > MFCFD_TIME_SWEEPS) return fail(MFCFD_EINVAL, "kernel timing selector %d", enable);
+    c->time_kernel = enable;
     c->tev_used = 0;
     c->tev_ms = 0.0;
     c->tev_launches = 0;

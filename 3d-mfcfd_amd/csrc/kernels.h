@@ -59,13 +59,18 @@ struct SweepAsync {
     int n_old;
     const int* old_src;         // copy sources read as old: dold is taken for them
     int n_old_src;
-    const int* wait_src;        // copy sources read as this sweep's delU: marked pending
+    const int* wait_src;        // copy sources read as this sweep's delU (slot j of dsrc: wait_src[j])
     int n_wait_src;
-    double* pub;                // [5n] published prim(U + delU) of the positions (completion marks)
+    double* pub;                // [5n] published prim(U + delU) of the positions (completion marks;
+                                // fine-grained memory)
     double* pold;               // [5n] prim(U + delU) before the sweep, for old entries
     double* dold;               // [5n] delU before the sweep, for copies from old sources (may be null)
+    double* dsrc;               // [5 n_wait_src] published delU of the copy sources read as this
+                                // sweep's value (fine-grained; may be null)
+    const int* src_slot;        // [n] slot of each point in dsrc, -1 if none (may be null)
     unsigned* ctr;              // [2] ticket counter, finished workgroups (zero between sweeps)
     int* stalls;                // waits that hit the spin limit
+    unsigned long long zero;    // 0: the operand of the fallback polling atomic adds
 };
 
 // setup: Cramer weights of every direction list (geometry only)
@@ -108,8 +113,10 @@ void launch_dinv_geometry(const DevGeom& g, double* dgeo, double power, hipStrea
 void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams& p,
                           const SweepSchedule& sch, hipStream_t st);
 // forward and backward sweeps without level barriers, then delUp = delUn
+// (ev: null, or begin/end events recorded around the forward and the
+// backward sweep kernel launches)
 void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
-                         const SweepAsync& bwd, int grid, hipStream_t st);
+                         const SweepAsync& bwd, int grid, hipStream_t st, const hipEvent_t* ev = nullptr);
 // resident workgroup count used for the sync-free sweeps
 int sweep_async_grid(int count, bool unit_weight, bool mixed);
 // setup: the least-squares weight of every entry of a sweep's plan

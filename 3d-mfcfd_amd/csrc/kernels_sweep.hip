@@ -41,16 +41,23 @@
 //                  and its published state prim(U_i + delU_i), computed once
 //                  here instead of once per entry that reads it.
 //
-// Memory model.  Inside k_sweep_async the only memory written is delU and
-// pub of the positions, each 64-bit value once, with agent-scope relaxed
-// atomic stores (sc1: written through to the coherent level, no dirty line
-// in any XCD's L2); the values it polls are read with agent-scope relaxed
-// atomic loads (sc1: never served from a possibly stale L2 line).  Each
-// value is checked on its own, so no ordering between locations is needed.
-// The pending marks are written with the same agent-scope atomic stores, so
-// no XCD's L2 ever holds a dirty line of these arrays that a later
-// write-back could lay over a result.  Everything else the kernel reads is
-// not written while it runs.
+// Memory model.  Inside k_sweep_async the memory other waves read while it
+// runs is pub (and dsrc, the published delU of the copy sources), each 64-bit
+// value written once.  These arrays are fine-grained device memory: hipMalloc's
+// coarse-grained memory is coherent only at kernel boundaries, and on MI355X
+// a value written there by one wave was measured still reading as pending,
+// through agent-scope atomic loads on another CU, a second after its writer
+// had finished.  A position publishes each value with a device-scope atomic
+// swap; a waiting lane reads with agent-scope atomic loads and, should a
+// value still read pending after kSpinAtomic polls, re-reads it with
+// device-scope atomic adds of zero, which are performed at the device's
+// coherence point (the zero comes from a kernel argument, so the compiler
+// cannot turn the add into a load).  A non-pending value is the final one:
+// the marks are written before the kernel starts and each location is
+// written once while it runs.  Each value is checked on its own, so no
+// ordering between locations is needed.  Everything else the kernel reads
+// is not written while it runs, and delU, which the later kernels read, is
+// ordered by the kernel boundary.
 //
 // Progress.  Every wait entry reads a point that executes earlier, so it was
 // taken earlier (or in the same round) by a wave that is running.  The lowest
@@ -82,31 +89,38 @@ constexpr int kSweepWaves = 4;
 constexpr int kSweepBlock = 64 * kSweepWaves;
 // about a second of polling; legitimate waits end within milliseconds
 constexpr int kSpinLimit = 1 << 20;
+// polls by atomic load before the atomic-add fallback
+constexpr int kSpinAtomic = 4096;
 
 // a signalling NaN with a payload: arithmetic only ever produces quiet NaNs
 constexpr unsigned long long kPending = 0x7FF4DEAD5EEDF00DULL;
 
+// publish one value: a device-scope atomic swap, performed at the coherence
+// point every XCD's atomics read
 __device__ __forceinline__ void store_value(double* p, double v) {
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_exchange(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// read five values written in this sweep, waiting until none is pending.
-// Gives up after the spin limit, or at once when another wait of this sweep
-// already gave up, so a broken schedule ends the kernel instead of stalling
-// it point by point.
-__device__ __forceinline__ void wait_read(const double* v5, double out[5], int* stalls) {
-    int spins = 0;
-    for (;;) {
-        bool ok = true;
+// read five values written in this sweep, waiting until none is pending:
+// plain agent-scope loads first (the value is usually there already), then
+// device-scope atomic adds of zero until every value has arrived.  Gives up
+// after the spin limit, or at once when another wait of this sweep already
+// gave up, so a broken schedule ends the kernel instead of stalling it point
+// by point.
+__device__ __forceinline__ void wait_read(double* v5, double out[5], int* stalls, unsigned long long zero) {
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(v5);
+    bool ok = true;
 #pragma unroll
-        for (int r = 0; r < 5; ++r) {
-            const unsigned long long b = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(v5 + r),
-                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            out[r] = __longlong_as_double((long long)b);
-            ok = ok && b != kPending;
-        }
-        if (ok) return;
+    for (int r = 0; r < 5; ++r) {
+        const unsigned long long b = __hip_atomic_load(w + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        out[r] = __longlong_as_double((long long)b);
+        ok = ok && b != kPending;
+    }
+    if (ok) return;
+    int spins = 0;
+#pragma clang loop unroll(disable)
+    for (;;) {
         ++spins;
         if ((spins & 255) == 0 && __hip_atomic_load(stalls, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
             return;
@@ -115,7 +129,25 @@ __device__ __forceinline__ void wait_read(const double* v5, double out[5], int* 
             return;
         }
         __builtin_amdgcn_s_sleep(1);
+        ok = true;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const unsigned long long b =
+                spins < kSpinAtomic
+                    ? __hip_atomic_load(w + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    : __hip_atomic_fetch_add(w + r, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            out[r] = __longlong_as_double((long long)b);
+            ok = ok && b != kPending;
+        }
+        if (ok) return;
     }
+}
+
+// a 64-bit value every lane of the wave holds alike, in scalar registers
+__device__ __forceinline__ long long uniform64(long long v) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)v);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return ((long long)hi << 32) | (long long)(unsigned)lo;
 }
 
 // one DPP move of a double (both 32-bit halves through v_mov_b32_dpp): a
@@ -220,9 +252,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep_snap(DevGeom g, DevState s, co
     }
 }
 
-// pending marks: pub of every position, delU of every copy source read as
-// this sweep's value; agent-scope atomic stores (no dirty L2 line)
-__global__ void __launch_bounds__(kBlock) k_sweep_mark(double* dU, SweepAsync a) {
+// pending marks: pub of every position and every slot of dsrc
+__global__ void __launch_bounds__(kBlock) k_sweep_mark(SweepAsync a) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n1 = 5LL * a.count;
     unsigned long long* dst;
@@ -230,8 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep_mark(double* dU, SweepAsync a)
         const long long t = e / 5;
         dst = reinterpret_cast<unsigned long long*>(a.pub) + 5LL * a.seq[t] + (e - 5 * t);
     } else if (e < n1 + 5LL * a.n_wait_src) {
-        const long long t = (e - n1) / 5;
-        dst = reinterpret_cast<unsigned long long*>(dU) + 5LL * a.wait_src[t] + (e - n1 - 5 * t);
+        dst = reinterpret_cast<unsigned long long*>(a.dsrc) + (e - n1);
     } else {
         return;
     }
@@ -253,29 +283,31 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
             sh_base[round & 1] = __hip_atomic_fetch_add(a.ctr, (unsigned)kSweepWaves, __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        const unsigned base = sh_base[round & 1];
+        // wave-uniform values are moved to scalar registers (readfirstlane):
+        // the position, the point and its entry range
+        const unsigned base = __builtin_amdgcn_readfirstlane(sh_base[round & 1]);
         if (base >= (unsigned)a.count) break;   // the same for the whole workgroup
-        const unsigned t = base + wave;
+        const unsigned t = __builtin_amdgcn_readfirstlane(base + wave);
         if (t >= (unsigned)a.count) continue;   // the last, partial round
-        const int i = a.seq[t];
-        const long long tb = a.take_ptr[t], te = a.take_ptr[t + 1];
+        const int i = __builtin_amdgcn_readfirstlane(a.seq[t]);
+        const long long tb = uniform64(a.take_ptr[t]), te = uniform64(a.take_ptr[t + 1]);
         const Frame<double> f = load_frame(g.frame, i);
         double res[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        const unsigned first = te > tb ? a.take[tb] : 0u;
+        const unsigned first = te > tb ? __builtin_amdgcn_readfirstlane(a.take[tb]) : 0u;
         if (te > tb && (first & 7u) == kTakeCopy) {
             // supersonic point: the nearest neighbour's delU
             // (implicit_forward_and_reverse_sweeps.f90:38-52, 69-83)
             if (lane == 0) {
                 const long long q = (long long)(first >> kTakeShift);
                 if (first & kTakeWait) {
-                    wait_read(dU + 5 * q, res, a.stalls);
+                    wait_read(a.dsrc + 5 * (long long)a.src_slot[q], res, a.stalls, a.zero);
                 } else {
 #pragma unroll
                     for (int r = 0; r < 5; ++r) res[r] = a.dold[5 * q + r];
                 }
             }
         } else {
-            const int cls = g.status[i];
+            const int cls = __builtin_amdgcn_readfirstlane(g.status[i]);
             const Frame<T> ft = frame_as<T>(f);
             // the point's own inputs of the final update, read before the wait
             double D_inv = 0.0, own[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -291,14 +323,12 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 const long long k = (long long)(packed >> kTakeShift);
                 const int d = (int)(packed & 7u);
                 const double wgt = a.wgt[e];
-                // the old flux first: it does not depend on this sweep
-                T prim_old[5], Gold[5];
-#pragma unroll
-                for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k + r]);
-                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
+                // the neighbour's state of this sweep first: nothing is held
+                // across the wait (a point's dependencies have mostly
+                // finished before its wave reaches them), then both fluxes
                 double pk[5];
                 if (packed & kTakeWait) {
-                    wait_read(a.pub + 5 * k, pk, a.stalls);
+                    wait_read(a.pub + 5 * k, pk, a.stalls, a.zero);
                 } else {
 #pragma unroll
                     for (int r = 0; r < 5; ++r) pk[r] = a.pold[5 * k + r];
@@ -307,6 +337,10 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
 #pragma unroll
                 for (int r = 0; r < 5; ++r) pt[r] = T(pk[r]);
                 flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
+                T prim_old[5], Gold[5];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k + r]);
+                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
 #pragma unroll
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }
@@ -325,12 +359,18 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
             double U[5], pp[5];
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
-                store_value(dU + 5 * (long long)i + r, res[r]);
+                dU[5 * (long long)i + r] = res[r];
                 U[r] = res[r] + s.U[5 * (long long)i + r];
             }
             conserved_to_prim(U, f, pp);
 #pragma unroll
             for (int r = 0; r < 5; ++r) store_value(a.pub + 5 * (long long)i + r, pp[r]);
+            // a copy source that a supersonic point reads as this sweep's value
+            const int slot = a.src_slot ? a.src_slot[i] : -1;
+            if (slot >= 0) {
+#pragma unroll
+                for (int r = 0; r < 5; ++r) store_value(a.dsrc + 5 * (long long)slot + r, res[r]);
+            }
         }
     }
     // the last workgroup out resets the counters for the next sweep
@@ -370,22 +410,24 @@ int pair_grid(int count) {
 
 template <bool FORWARD, bool UW, typename T>
 void launch_one(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& a, int grid,
-                hipStream_t st) {
+                hipStream_t st, const hipEvent_t* ev) {
     double* dU = FORWARD ? s.delUp : s.delUn;
     const long long n_snap = (long long)a.n_old + a.n_old_src;
     if (n_snap > 0)
         hipLaunchKernelGGL(k_sweep_snap, dim3(grid_for(n_snap, kBlock)), dim3(kBlock), 0, st, g, s,
                            (const double*)dU, a);
     const long long n_mark = 5LL * ((long long)a.count + a.n_wait_src);
-    hipLaunchKernelGGL(k_sweep_mark, dim3(grid_for(n_mark, kBlock)), dim3(kBlock), 0, st, dU, a);
+    hipLaunchKernelGGL(k_sweep_mark, dim3(grid_for(n_mark, kBlock)), dim3(kBlock), 0, st, a);
+    if (ev) hipEventRecord(ev[0], st);
     hipLaunchKernelGGL((k_sweep_async<FORWARD, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, a);
+    if (ev) hipEventRecord(ev[1], st);
 }
 
 template <bool UW, typename T>
 void launch_pair(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
-                 const SweepAsync& bwd, int grid, hipStream_t st) {
-    launch_one<true, UW, T>(g, s, p, fwd, grid, st);
-    launch_one<false, UW, T>(g, s, p, bwd, grid, st);
+                 const SweepAsync& bwd, int grid, hipStream_t st, const hipEvent_t* ev) {
+    launch_one<true, UW, T>(g, s, p, fwd, grid, st, ev);
+    launch_one<false, UW, T>(g, s, p, bwd, grid, st, ev ? ev + 2 : nullptr);
 }
 
 }  // namespace
@@ -405,18 +447,18 @@ void launch_sweep_weights(const DevGeom& g, const SweepAsync& a, double power, h
 }
 
 void launch_sweeps_async(const DevGeom& g, const DevState& s, const FlowParams& p, const SweepAsync& fwd,
-                         const SweepAsync& bwd, int grid, hipStream_t st) {
+                         const SweepAsync& bwd, int grid, hipStream_t st, const hipEvent_t* ev) {
     if (fwd.count > 0 && grid > 0) {
         if (p.mixed) {
             if (p.power == 0.0)
-                launch_pair<true, float>(g, s, p, fwd, bwd, grid, st);
+                launch_pair<true, float>(g, s, p, fwd, bwd, grid, st, ev);
             else
-                launch_pair<false, float>(g, s, p, fwd, bwd, grid, st);
+                launch_pair<false, float>(g, s, p, fwd, bwd, grid, st, ev);
         } else {
             if (p.power == 0.0)
-                launch_pair<true, double>(g, s, p, fwd, bwd, grid, st);
+                launch_pair<true, double>(g, s, p, fwd, bwd, grid, st, ev);
             else
-                launch_pair<false, double>(g, s, p, fwd, bwd, grid, st);
+                launch_pair<false, double>(g, s, p, fwd, bwd, grid, st, ev);
         }
     }
     // implicit_forward_and_reverse_sweeps.f90:86-88; the alternative driver

@@ -40,6 +40,26 @@ def flux_bytes_per_point(status, split_counts, nbhs):
                                                      np.where(status == 2, outer, 0)))
 
 
+def sweep_entries(status, split_counts, nbhs):
+    """List entries one sweep walks per point: every direction list (wall:
+    four split lists and conn for z), whether the entry is taken or not."""
+    status = np.asarray(status)
+    cnt = np.asarray(split_counts, dtype=np.int64)
+    nbhs = np.asarray(nbhs, dtype=np.int64)
+    return np.where(status == 0, cnt.sum(axis=0), cnt[:4].sum(axis=0) + nbhs)
+
+
+def sweep_bytes_per_point(status, split_counts, nbhs):
+    """Algorithmic bytes of one sweep (LdelU or UdelU pass) for each point:
+    index 4, pos 32, frame 72, alias 4, dinv 8, list counts 24, per list
+    entry the entry and the neighbour's alias (8), flux_res or delUp 40
+    read, the new delta 40 written = 224 + 8*entries; 0 for points the
+    canonical sweeps do not visit (outer)."""
+    status = np.asarray(status)
+    swept = (status == 0) | (status == 1)
+    return np.where(swept, 224 + 8 * sweep_entries(status, split_counts, nbhs), 0)
+
+
 def iteration_bytes(status, split_counts, nbhs, mode="explicit"):
     """Per-kernel algorithmic bytes of one whole iteration (dict of totals)."""
     status = np.asarray(status)

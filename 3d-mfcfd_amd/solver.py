@@ -317,9 +317,12 @@ class Solver:
         _check(lib().mfcfd_point_class_map(self._ctx, out.ctypes.data_as(_ip)))
         return out
 
-    def kernel_timing(self, enable=True):
-        """Record HIP events around every fused interior flux launch."""
-        _check(lib().mfcfd_kernel_timing(self._ctx, 1 if enable else 0))
+    def kernel_timing(self, enable=True, kernel="flux"):
+        """Record HIP events around every launch of one kernel: "flux" (the
+        fused interior flux kernel) or "sweeps" (each sync-free sweep kernel
+        launch, forward and backward)."""
+        sel = {"flux": 1, "sweeps": 2}[kernel] if enable else 0
+        _check(lib().mfcfd_kernel_timing(self._ctx, sel))
 
     def kernel_time(self):
         """(summed ms, launches) of the fused interior flux kernel since

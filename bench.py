@@ -1,30 +1,43 @@
 """Benchmark: Mpoint-iterations/s of the q-LSKUM pseudo-time iteration.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2_2m]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3_8m]
 
 A step is one fpi_solver iteration over the whole cloud.  Default workload:
-the 2M-point ellipsoid cloud (120k wall, 40k outer, 1.84M interior, k=25),
-FP64 explicit, Mach 0.84, AoA 3.06 deg, CFL 0.2, freestream start; the cloud
-is generated from its seed and resident in HBM before the timed region.
-W warmup iterations, then K timed iterations, each side bracketed by a
-device synchronize.
+the 8M-point ellipsoid cloud (400k wall, 120k outer, 7.48M interior, k=25,
+Morton order), FP64 implicit LU-SGS, Mach 0.84, AoA 3.06 deg, CFL 50, from
+the freestream, generated from its seed and resident in HBM before the
+timed region; its mixed-precision twin is reported beside it.
+
+Only finite states are timed.  At CFL 50 the reference's own canonical
+implicit pairing diverges (tests/golden/cfl50_20k_history.npz: the far
+field grows until the pressures turn non-finite), and on the 8M cloud the
+FP64 run turns non-finite at iteration 14, the mixed one at iteration 9.  So
+the state after W warmup iterations is saved, and the K timed iterations run
+in blocks of at most --block iterations, each block starting from that saved
+state (restored by host copies outside the timed region): with the
+defaults, iterations 4..7 five times.  Explicit configurations (c2_2m) time
+their K iterations in one block.
 
 Reported beside the throughput:
-  roofline      the fused interior flux-residual kernel (the dominant
-                kernel): algorithmic bytes per launch (the reference's
-                eval_flux_residual traffic, 388 + 4*(6 + list entries) bytes
-                per point, for the points that launch processes) / its mean
-                launch time, from HIP events recorded on the library's stream
-                around every launch of the timed iterations, against 8 TB/s;
-                traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE and
-                WRITE_SIZE when a summary exists in profiles/;
+  roofline      the dominant kernel -- the sync-free sweep kernel for an
+                implicit run (algorithmic bytes per launch: 224 + 8 per list
+                entry for every swept point, roofline.sweep_bytes_per_point),
+                the fused interior flux kernel for an explicit one -- divided
+                by its mean launch time from HIP events recorded on the
+                library's stream around every launch of the timed
+                iterations, against 8 TB/s; traffic = HBM bytes per launch
+                from the rocprofv3 FETCH_SIZE and WRITE_SIZE summary in
+                profiles/ when one exists, valu from the rocprofv3 SQ
+                counter summary when one exists;
   cpu_baseline  the C restatement of the reference (bit-identical to the
-                reference Fortran on the golden fixtures), one core, one
-                iteration of the same cloud from the GPU's state.
-  mixed         the same W + K iterations in the mixed-precision form (FP32
+                reference Fortran on the golden fixtures) on one core, on a
+                bounded sample of the same workload;
+  mixed         the same iterations in the mixed-precision form (FP32
                 q-variables, gradients and split fluxes): its rate, its
-                speed-up over FP64 and its wall-Cp error against the FP64
-                state after the same iterations.
+                speed-up over FP64, and its wall Cp, CL/CD and residual drop
+                against the FP64 state after the same iterations;
+  c2_2m         (implicit default only) the 2M-point explicit FP64
+                configuration's rate, roofline and mixed speed-up.
 """
 
 from __future__ import annotations
@@ -53,35 +66,42 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(config):
-    """HBM bytes per interior flux launch from a committed rocprof summary."""
-    path = os.path.join(ROOT, "profiles", f"flux_interior_{config}_traffic.txt")
-    if not os.path.exists(path):
-        return None
+def read_profile_values(name):
+    """key = value lines of a committed rocprof summary in profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
     vals = {}
+    if not os.path.exists(path):
+        return vals
     with open(path) as f:
         for line in f:
             if "=" in line and not line.lstrip().startswith("#"):
                 k, v = line.split("=", 1)
-                vals[k.strip()] = v.strip()
-    try:
-        return float(vals["hbm_bytes_per_launch"])
-    except (KeyError, ValueError):
-        return None
+                try:
+                    vals[k.strip()] = float(v.strip())
+                except ValueError:
+                    pass
+    return vals
 
 
-def cpu_baseline(cloud, cfg, state, t):
+def cpu_baseline_sample(cfg, n_sample):
+    """The C oracle on one core: a bounded sample of the configuration --
+    the same generator and flow on n_sample points, timed over its first two
+    iterations from the freestream."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
-    o = orc.Oracle(cloud, cfg["mach"], cfg["aoa"], cfg["cfl"], prim=state["prim"])
-    o.delUp[:] = state["delUp"]
-    o.delUn[:] = state["delUn"]
+    n_wall = int(round(n_sample * cfg["n_wall"] / (cfg["n_wall"] + cfg["n_outer"] + cfg["n_interior"])))
+    n_outer = int(round(n_sample * cfg["n_outer"] / (cfg["n_wall"] + cfg["n_outer"] + cfg["n_interior"])))
+    cl = pc.ellipsoid_cloud(n_wall, n_outer, n_sample - n_wall - n_outer, seed=cfg["seed"], order=cfg["order"],
+                            workers=16)
+    prim0 = np.tile(pc.freestream_prim(cfg["mach"], cfg["aoa"]), (cl.n, 1))
+    o = orc.Oracle(cl, cfg["mach"], cfg["aoa"], cfg["cfl"], prim=prim0)
     t0 = time.perf_counter()
-    o.fpi_solver(t, cfg["mode"] == "implicit")
+    o.run(1, 2, cfg["mode"])
     dt = time.perf_counter() - t0
-    return {"value": cloud.n / dt / 1e6, "unit": "Mpoint-iter/s", "cores": 1, "kind": "port",
-            "sample": f"1 iteration (t={t}) of the same {cloud.n}-point cloud from the GPU state, "
-                      f"C restatement of the reference on one host core, {dt:.1f} s"}
+    return {"value": round(2 * cl.n / dt / 1e6, 5), "unit": "Mpoint-iter/s", "cores": 1, "kind": "port",
+            "sample": f"iterations 1-2 of a {cl.n}-point cloud from the same generator and flow "
+                      f"({cfg['mode']}, CFL {cfg['cfl']}), C restatement of the reference on one host core, "
+                      f"{dt:.1f} s"}
 
 
 def copy_probe(mib):
@@ -103,6 +123,174 @@ def class_counts(cls, status):
     return {"fused_interior": int(np.sum(cls == 0)), "fused_general": int(np.sum(cls == 1)),
             "fused_general_interior": int(np.sum((cls == 1) & (status == pc.STATUS_INTERIOR))),
             "listwise": int(np.sum(cls == 2))}
+
+
+STATE = ("prim", "delUp", "delUn")
+
+
+def run_leg(cloud, cfg, split, alias, precision, W, K, block, tune, phase_iters, n_cp):
+    """W warmup iterations, then K timed iterations in blocks of at most
+    `block`, each from the state after the warmup; then the state at
+    iteration n_cp (from the same saved state) for the accuracy comparison."""
+    implicit = cfg["mode"] != "explicit"
+    s = solver.Solver(cloud, cfg["mach"], cfg["aoa"], cfg["cfl"], mode=cfg["mode"], precision=precision,
+                      split=split, alias=alias)
+    s.upload("prim", np.tile(pc.freestream_prim(cfg["mach"], cfg["aoa"]), (cloud.n, 1)))
+    if implicit:
+        log(f"{precision}: sweep plan {s.sweep_info()}")
+    hist0 = np.zeros(0)
+    if W > 0:
+        hist0, _ = s.run(1, W)
+    s.synchronize()
+    saved = {nm: s.download(nm) for nm in STATE}
+
+    def restore():
+        for nm in STATE:
+            s.upload(nm, saved[nm])
+        s.synchronize()
+
+    tuning = None
+    if tune:
+        log(f"{precision}: autotune")
+        tuning = s.autotune()
+        restore()
+    blocks = [K] if not implicit else [min(block, K - b0) for b0 in range(0, K, block)]
+    elapsed = 0.0
+    kernel = "sweeps" if implicit else "flux"
+    s.kernel_timing(True, kernel)
+    rn_all = []
+    for bi, nb in enumerate(blocks):
+        if bi > 0:
+            restore()
+        s.synchronize()
+        t0 = time.perf_counter()
+        rn, _ = s.run(W + 1, nb)
+        s.synchronize()
+        elapsed += time.perf_counter() - t0
+        rn_all.append(rn)
+    k_ms, k_launches = s.kernel_time()
+    s.kernel_timing(False)
+    log(f"{precision}: timed {K} iterations in blocks {blocks}: {elapsed / K * 1000:.3f} ms/iter, "
+        f"{cloud.n * K / elapsed / 1e6:.2f} Mpoint-iter/s; {kernel} kernel {k_launches} launches, "
+        f"mean {k_ms / max(k_launches, 1):.4f} ms")
+    phases = None
+    done = W + K                       # explicit: iterations run so far
+    if phase_iters > 0:
+        if implicit:
+            restore()
+            phases = s.time_phases(W + 1, min(phase_iters, block))
+        else:
+            phases = s.time_phases(done + 1, phase_iters)
+            done += phase_iters
+    # the comparison state (from the saved state for implicit runs)
+    if implicit:
+        restore()
+        rn_cp, _ = s.run(W + 1, n_cp - W)
+    elif n_cp > done:
+        s.run(done + 1, n_cp - done)
+    prim = s.download("prim")
+    cls = s.point_class_map()
+    hist = np.concatenate([hist0, rn_cp]) if implicit else None
+    s.close()
+    return dict(elapsed=elapsed, ms=elapsed / K * 1000.0, value=cloud.n * K / elapsed / 1e6, tuning=tuning,
+                kernel_ms=k_ms / max(k_launches, 1), kernel_launches=k_launches, phases=phases, prim=prim,
+                cls=cls, blocks=blocks, rn=np.concatenate(rn_all), hist=hist)
+
+
+def bench_config(args, name, steps, warmup, mixed=True, cpu=True):
+    cfg = pc.CONFIGS[name]
+    implicit = cfg["mode"] != "explicit"
+    log(f"generating {name}")
+    t0 = time.perf_counter()
+    cloud = pc.config_cloud(name, workers=16)
+    log(f"cloud {cloud.counts()} in {time.perf_counter() - t0:.1f} s")
+    cnt, lists = solver.split_stencils(cloud)
+    alias = solver.alias_order(cloud)
+    W, K = warmup, steps
+    block = args.block if implicit else K
+    # implicit: compare at the last iteration of a block; explicit: at the
+    # configuration's own iteration count
+    n_cp = (W + block) if implicit else (args.cp_iters if args.cp_iters > 0 else cfg["iters"])
+    n_cp = max(n_cp, W + 1) if implicit else max(n_cp, W + K + args.phase_iters)
+    legs = {}
+    for precision in (["fp64", "mixed"] if mixed else ["fp64"]):
+        legs[precision] = run_leg(cloud, cfg, (cnt, lists), alias, precision, W, K, block, not args.no_tune,
+                                  args.phase_iters, n_cp)
+    f64 = legs["fp64"]
+    status = cloud.status
+    if implicit:
+        per_pt = roofline.sweep_bytes_per_point(status, cnt, cloud.nbhs)
+        k_bytes = int(per_pt.sum())
+        kname = "k_sweep_async<FORWARD, UW, double> (forward and backward sync-free LU-SGS sweeps)"
+        prof = read_profile_values(f"sweep_{name}_traffic.txt")
+        valu = read_profile_values(f"sweep_{name}_valu.txt")
+    else:
+        k_bytes = int(roofline.flux_bytes_per_point(status, cnt, cloud.nbhs)[f64["cls"] == 0].sum())
+        kname = "k_flux_fused<0, true, double> (interior points on the fused path)"
+        prof = read_profile_values(f"flux_interior_{name}_traffic.txt")
+        valu = read_profile_values(f"flux_interior_{name}_valu.txt")
+    achieved = k_bytes / (f64["kernel_ms"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / roofline.HBM_PEAK_GBS, 5),
+            "traffic": prof.get("hbm_bytes_per_launch"), "kernel": kname, "bytes_per_launch": k_bytes,
+            "launch_ms": round(f64["kernel_ms"], 4), "launches_timed": f64["kernel_launches"]}
+    if valu:
+        roof["valu"] = {k: valu[k] for k in sorted(valu)}
+    by = roofline.iteration_bytes(status, cnt, cloud.nbhs, cfg["mode"])
+    step_gbs = by["total"] / (f64["ms"] * 1e-3) / 1e9
+    mixed_out = None
+    if "mixed" in legs:
+        mx = legs["mixed"]
+        wall = status == pc.STATUS_WALL
+        cp64 = postprocess.wall_cp(f64["prim"][wall], cfg["mach"])
+        cpm = postprocess.wall_cp(mx["prim"][wall], cfg["mach"])
+        areas = postprocess.wall_areas(cloud)
+        fc64 = postprocess.force_coefficients(cloud, f64["prim"], cfg["mach"], cfg["aoa"], areas=areas)
+        fcm = postprocess.force_coefficients(cloud, mx["prim"], cfg["mach"], cfg["aoa"], areas=areas)
+        mixed_out = {"value": round(mx["value"], 3), "ms_per_step": round(mx["ms"], 4),
+                     "speedup_vs_fp64": round(mx["value"] / f64["value"], 3),
+                     "wall_cp_err_vs_fp64": float(np.max(np.abs(cpm - cp64))),
+                     "compared_at_iteration": n_cp,
+                     "CL": [fc64["CL"], fcm["CL"]], "CD": [fc64["CD"], fcm["CD"]],
+                     "CL_err": abs(fcm["CL"] - fc64["CL"]), "CD_err": abs(fcm["CD"] - fc64["CD"]),
+                     "nonfinite_points": [int((~np.isfinite(f64["prim"])).any(1).sum()),
+                                          int((~np.isfinite(mx["prim"])).any(1).sum())],
+                     "kernel_ms": round(mx["kernel_ms"], 4),
+                     "phases_ms": {k: round(v, 4) for k, v in mx["phases"].items()} if mx["phases"] else None,
+                     "tuning": mx["tuning"]}
+        if implicit:
+            # residual drop over the compared iterations (res_new at n_cp over
+            # res_old, the residual of iteration 2: fpi_solver.f90:38-43)
+            d64 = f64["hist"][n_cp - 1] / f64["hist"][1]
+            dm = mx["hist"][n_cp - 1] / mx["hist"][1]
+            mixed_out["residual_drop"] = [float(d64), float(dm)]
+            mixed_out["residual_drop_err_rel"] = float(abs(dm - d64) / abs(d64))
+        log(f"mixed: {mixed_out}")
+    cpu_out = None
+    if cpu:
+        log("cpu baseline: the oracle on one core, bounded sample")
+        cpu_out = cpu_baseline_sample(cfg, args.cpu_sample_points if implicit else 250000)
+        log(f"cpu baseline {cpu_out}")
+    timing = (f"warmup iterations 1-{W}, state saved; timed iterations {W + 1}-{W + block} "
+              f"{len(f64['blocks'])} times (blocks {f64['blocks']}), each from the saved state "
+              f"(restores outside the timed region)") if implicit else f"iterations {W + 1}-{W + K}"
+    return {
+        "metric": METRIC, "value": round(f64["value"], 3), "unit": "Mpoint-iter/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(f64["ms"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{name}: ellipsoid cloud {cloud.n} points (wall {cfg['n_wall']}, outer "
+                               f"{cfg['n_outer']}, interior {cfg['n_interior']}, k=25, {cfg['order']} order), "
+                               f"FP64 {cfg['mode']}, Mach {cfg['mach']}, AoA {cfg['aoa']} deg, CFL {cfg['cfl']}",
+                   "points": cloud.n, "mode": cfg["mode"], "precision": "fp64", "timing": timing},
+        "roofline": roof,
+        "cpu_baseline": cpu_out,
+        "mixed": mixed_out,
+        "tuning": f64["tuning"],
+        "phases_ms": {k: round(v, 4) for k, v in f64["phases"].items()} if f64["phases"] else None,
+        "point_classes": class_counts(f64["cls"], status),
+        "step_algorithmic_gbs": round(step_gbs, 2),
+        "step_hbm_frac": round(step_gbs / roofline.HBM_PEAK_GBS, 5),
+    }
 
 
 def bench_batch(args):
@@ -192,16 +380,20 @@ def bench_batch(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c2_2m", choices=sorted(pc.CONFIGS) + ["c5_batch"])
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c3_8m", choices=sorted(pc.CONFIGS) + ["c5_batch"])
+    ap.add_argument("--block", type=int, default=4,
+                    help="implicit runs: timed iterations per block, each block from the saved state")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
-    ap.add_argument("--phase-iters", type=int, default=3)
+    ap.add_argument("--no-extra", action="store_true", help="skip the 2M explicit line of the default run")
+    ap.add_argument("--cpu-sample-points", type=int, default=250000)
+    ap.add_argument("--phase-iters", type=int, default=2)
     ap.add_argument("--cp-iters", type=int, default=0,
-                    help="iterations at which FP64 and mixed wall Cp are compared (default: the "
-                         "configuration's own count for explicit runs, 6 for implicit ones)")
+                    help="explicit runs: iteration at which FP64 and mixed wall Cp are compared "
+                         "(default: the configuration's own count)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -212,150 +404,18 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs "
               f"the q-LSKUM iteration on a single MI355X)", file=sys.stderr)
         sys.exit(2)
-
     if args.config == "c5_batch":
         bench_batch(args)
         return
-    cfg = pc.CONFIGS[args.config]
-    log(f"generating {args.config}")
-    t0 = time.perf_counter()
-    cloud = pc.config_cloud(args.config, workers=16)
-    log(f"cloud {cloud.counts()} in {time.perf_counter() - t0:.1f} s")
-    prim0 = np.tile(pc.freestream_prim(cfg["mach"], cfg["aoa"]), (cloud.n, 1))
-    cnt, lists = solver.split_stencils(cloud)
-    alias = solver.alias_order(cloud)
-    s = solver.Solver(cloud, cfg["mach"], cfg["aoa"], cfg["cfl"], mode=cfg["mode"],
-                      split=(cnt, lists), alias=alias)
-    s.upload("prim", prim0)
-    log(f"bound; sweep schedule {s.sweep_info()}")
-    W, K = args.warmup, args.steps
-    log("warmup")
-    if W > 0:
-        s.run(1, W)
-    s.synchronize()
-    # shapes tuned on the state the timed iterations start from
-    tuning = None
-    if not args.no_tune:
-        log("autotune")
-        tuning = s.autotune()
-        log(f"autotune: {tuning}")
-    log(f"warmup {W} done; timed iterations")
-    s.kernel_timing(True)
-    t0 = time.perf_counter()
-    rn, _ = s.run(W + 1, K)
-    s.synchronize()
-    elapsed = time.perf_counter() - t0
-    flux_total_ms, flux_launches = s.kernel_time()
-    s.kernel_timing(False)
-    ms_per_step = elapsed / K * 1000.0
-    value = cloud.n * K / elapsed / 1e6
-    log(f"timed {K} iterations: {ms_per_step:.3f} ms/iter, {value:.2f} Mpoint-iter/s, "
-        f"res_new {rn[0]:.6e} -> {rn[-1]:.6e}")
-
-    phases = s.time_phases(W + K + 1, args.phase_iters)
-    log("phases (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()))
-    by = roofline.iteration_bytes(cloud.status, cnt, cloud.nbhs, cfg["mode"])
-    cls = s.point_class_map()
-    flux_bytes = int(roofline.flux_bytes_per_point(cloud.status, cnt, cloud.nbhs)[cls == 0].sum())
-    flux_ms = flux_total_ms / max(flux_launches, 1)
-    log(f"fused interior flux kernel: {flux_launches} launches, mean {flux_ms:.4f} ms, "
-        f"{int(np.sum(cls == 0))} points, {flux_bytes} algorithmic bytes per launch")
-    achieved = flux_bytes / (flux_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 5),
-            "traffic": read_traffic(args.config),
-            "kernel": "k_flux_fused<0,true> (interior points on the fused path)",
-            "bytes_per_launch": flux_bytes, "launch_ms": round(flux_ms, 4),
-            "launches_timed": flux_launches}
-    step_gbs = by["total"] / (ms_per_step * 1e-3) / 1e9
-
-    # accuracy: both forms from the same start to the same iteration count
-    n_done = W + K + args.phase_iters
-    # Implicit runs compare at iteration 6: the reference updates outer
-    # points explicitly (U = Ubar - dt*R) with the implicit run's CFL-50 local
-    # step, and that update amplifies a far-field disturbance about 1.8x per
-    # iteration, in FP64 and in the oracle alike.  On the 8M cloud the FP64
-    # pressures turn non-finite from iteration 14, the mixed form's (seeded
-    # at FP32 rounding) from iteration 9.
-    n_cp = args.cp_iters if args.cp_iters > 0 else (cfg["iters"] if cfg["mode"] == "explicit" else 6)
-    n_cp = max(n_cp, n_done)
-    if not args.no_mixed and n_cp > n_done:
-        log(f"FP64 to iteration {n_cp} for the wall-Cp comparison")
-        s.run(n_done + 1, n_cp - n_done)
-    else:
-        n_cp = n_done
-    t_next = n_cp + 1
-    prim_fp64 = s.download("prim")
-    mixed = None
-    if not args.no_mixed:
-        # same start, same W + K iterations, mixed precision
-        m = solver.Solver(cloud, cfg["mach"], cfg["aoa"], cfg["cfl"], mode=cfg["mode"],
-                          precision="mixed", split=(cnt, lists), alias=alias)
-        m.upload("prim", prim0)
-        log("mixed leg: warmup")
-        if W > 0:
-            m.run(1, W)
-        m.synchronize()
-        log("mixed leg: autotune")
-        tuning_m = m.autotune() if not args.no_tune else None
-        log("mixed leg: timed iterations")
-        t0 = time.perf_counter()
-        rn_m, _ = m.run(W + 1, K)
-        m.synchronize()
-        el_m = time.perf_counter() - t0
-        phases_m = m.time_phases(W + K + 1, args.phase_iters) if args.phase_iters > 0 else None
-        if n_cp > n_done:
-            log(f"mixed to iteration {n_cp} for the wall-Cp comparison")
-            m.run(n_done + 1, n_cp - n_done)
-        prim_m = m.download("prim")
-        m.close()
-        wall = cloud.status == pc.STATUS_WALL
-        cp64 = postprocess.wall_cp(prim_fp64[wall], cfg["mach"])
-        cpm = postprocess.wall_cp(prim_m[wall], cfg["mach"])
-        v_m = cloud.n * K / el_m / 1e6
-        mixed = {"value": round(v_m, 3), "ms_per_step": round(el_m / K * 1000.0, 4),
-                 "speedup_vs_fp64": round(v_m / value, 3),
-                 # Cp is already normalised by the dynamic pressure: the
-                 # absolute difference is the error; the relative one is
-                 # only meaningful once the wall flow has developed
-                 "wall_cp_err_vs_fp64": float(np.max(np.abs(cpm - cp64))),
-                 "wall_cp_iterations": n_cp,
-                 "nonfinite_points": [int((~np.isfinite(prim_fp64)).any(1).sum()),
-                                      int((~np.isfinite(prim_m)).any(1).sum())],
-                 "wall_cp_err_rel_max_cp": float(np.max(np.abs(cpm - cp64)) / np.max(np.abs(cp64))),
-                 "res_new_last": float(rn_m[-1]), "res_new_last_fp64": float(rn[-1]),
-                 "phases_ms": {k: round(v, 4) for k, v in phases_m.items()} if phases_m else None,
-                 "tuning": tuning_m}
-        log(f"mixed: {mixed}")
-
-    cpu = None
-    if not args.no_cpu:
-        state = {nm: s.download(nm) for nm in ("prim", "delUp", "delUn")}
-        s.close()
-        log("cpu baseline: one oracle iteration on one core")
-        cpu = cpu_baseline(cloud, cfg, state, t_next)
-        log(f"cpu baseline {cpu['value']:.4f} Mpoint-iter/s")
-    else:
-        s.close()
-
-    out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "Mpoint-iter/s", "n_gpus": 1,
-        "steps": K, "warmup": W, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.config}: ellipsoid cloud {cloud.n} points "
-                               f"(wall {cfg['n_wall']}, outer {cfg['n_outer']}, interior "
-                               f"{cfg['n_interior']}, k=25), FP64 {cfg['mode']}, Mach "
-                               f"{cfg['mach']}, AoA {cfg['aoa']} deg, CFL {cfg['cfl']}",
-                   "points": cloud.n, "mode": cfg["mode"], "precision": "fp64"},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "mixed": mixed,
-        "tuning": tuning,
-        "phases_ms": {k: round(v, 4) for k, v in phases.items()},
-        "point_classes": class_counts(cls, cloud.status),
-        "step_algorithmic_gbs": round(step_gbs, 2),
-        "step_hbm_frac": round(step_gbs / roofline.HBM_PEAK_GBS, 5),
-    }
+    out = bench_config(args, args.config, args.steps, args.warmup, mixed=not args.no_mixed,
+                       cpu=not args.no_cpu)
+    if args.config == "c3_8m" and not args.no_extra:
+        c2 = bench_config(args, "c2_2m", args.steps, 5, mixed=not args.no_mixed, cpu=False)
+        out["c2_2m"] = {k: c2[k] for k in ("value", "ms_per_step", "roofline", "step_hbm_frac")}
+        out["c2_2m"]["workload"] = c2["config"]["workload"]
+        if c2["mixed"]:
+            out["c2_2m"]["mixed"] = {k: c2["mixed"][k] for k in ("value", "speedup_vs_fp64",
+                                                                 "wall_cp_err_vs_fp64", "CL_err", "CD_err")}
     print(json.dumps(out), flush=True)
 
 

@@ -259,9 +259,15 @@ int mfcfd_point_classes(mfcfd_ctx *ctx, int *n_fused_interior, int *n_fused_wall
 int mfcfd_point_class_map(mfcfd_ctx *ctx, int *cls);
 
 /* diagnostics: with timing enabled, every iteration records HIP events on
- * the context's stream around the fused interior flux launch (the dominant
- * kernel); mfcfd_kernel_time returns the summed milliseconds and the number
- * of timed launches since mfcfd_kernel_timing was last called */
+ * the context's stream around the launches of one kernel:
+ *   MFCFD_TIME_FLUX_INTERIOR  the fused interior flux kernel (the explicit
+ *                             form's dominant kernel);
+ *   MFCFD_TIME_SWEEPS         each sync-free sweep kernel launch (forward
+ *                             and backward; the implicit form's dominant one).
+ * mfcfd_kernel_time returns the summed milliseconds and the number of timed
+ * launches since mfcfd_kernel_timing was last called (0 disables timing) */
+#define MFCFD_TIME_FLUX_INTERIOR 1
+#define MFCFD_TIME_SWEEPS 2
 int mfcfd_kernel_timing(mfcfd_ctx *ctx, int enable);
 int mfcfd_kernel_time(mfcfd_ctx *ctx, double *total_ms, long long *launches);
 

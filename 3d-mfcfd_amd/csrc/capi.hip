@@ -30,15 +30,13 @@ using namespace mfcfd;
 
 namespace {
 
-thread_local std::string g_last_error;
+thread_local char g_last_error[512];
 
 int fail(int code, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    std::vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
     va_end(ap);
-    g_last_error = buf;
     return code;
 }
 
@@ -152,6 +150,7 @@ struct mfcfd_ctx {
     // residual
     DevBuf<double> partial;
     DevBuf<int> partial_pos;
+    DevBuf<int> stencil_bad;   // bind-time singular-stencil flag
     int n_partial = 0;
     DevBuf<ResidualRecord> rec;
     DevBuf<double> hist;
@@ -207,6 +206,9 @@ struct mfcfd_ctx {
     // one iteration captured as a HIP graph (mfcfd_run_graph)
     hipGraphExec_t graph_exec = nullptr;
     bool graph_stale = true;
+    // U, q (q32) and tsden hold compute_conserved_vector and q_variables of
+    // the current prim, computed by the last iteration's state update
+    bool q_fresh = false;
     int sweep_grid = 0, sweep_grid_mixed = 0;
     // bookkeeping of fpi_solver.f90:38-43
     double res_old = 0.0, res_new = 0.0, residue = 0.0, max_res = 0.0, sum_res_sqr = 0.0;
@@ -268,7 +270,7 @@ struct mfcfd_ctx {
         DevBuf<int>* ints[] = {&status, &nbhs, &conn, &dir_cnt, &lst, &alias, &respos, &perm, &nwalk,
                                &idx_interior,
                                &idx_wall, &idx_outer, &idx_sup_in, &idx_sup_out, &idx_fused_interior,
-                               &idx_fused_wall, &idx_generic, &partial_pos};
+                               &idx_fused_wall, &idx_generic, &partial_pos, &stencil_bad};
         for (auto* b : ints) b->release();
         sw_fwd.release();
         sw_bwd.release();
@@ -573,8 +575,8 @@ int build_plan(mfcfd_ctx* c, int form) {
     if (copies && c->dold.n == 0 && c->dold.alloc(5ULL * n) != hipSuccess)
         return fail(MFCFD_ENOMEM, "sweep values");
     if (c->sweep_ctr.n == 0) {
-        if (c->sweep_ctr.alloc(2) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep counters");
-        HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, 2 * sizeof(unsigned), st));
+        if (c->sweep_ctr.alloc(kSweepCounters) != hipSuccess) return fail(MFCFD_ENOMEM, "sweep counters");
+        HIP_TRY(hipMemsetAsync(c->sweep_ctr.p, 0, kSweepCounters * sizeof(unsigned), st));
     }
     launch_sweep_weights(c->geom(), sweep_args(c, c->sw_fwd), c->fp.power, st);
     launch_sweep_weights(c->geom(), sweep_args(c, c->sw_bwd), c->fp.power, st);
@@ -594,26 +596,35 @@ int prepare_plan(mfcfd_ctx* c) {
     return build_plan(c, c->params.mode);
 }
 
+// next_q: the pass also computes the next iteration's U, q and time-step
+// denominators (an iteration that finds q_fresh skips its first step)
 int do_state_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr,
-                    bool with_explicit_sweep = false) {
+                    bool with_explicit_sweep = false, bool next_q = false) {
     launch_state_update(c->geom(), c->state(), c->fp, c->tl, c->partial.p, c->partial_pos.p, c->n_partial,
-                        c->rec.p, hist, hist_index, c->stream, ir, with_explicit_sweep);
+                        c->rec.p, hist, hist_index, c->stream, ir, with_explicit_sweep, next_q);
+    c->q_fresh = next_q;
     return launch_error("state update");
 }
 
 // the sweeps and the state update of one iteration; the explicit form's
 // sweep runs inside the state update's pass
-int do_sweeps_and_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
-    if (c->params.mode == MFCFD_EXPLICIT) return do_state_update(c, hist, hist_index, ir, true);
+int do_sweeps_and_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr,
+                         bool next_q = false) {
+    if (c->params.mode == MFCFD_EXPLICIT) return do_state_update(c, hist, hist_index, ir, true, next_q);
     int rc = do_sweeps(c);
     if (rc) return rc;
-    return do_state_update(c, hist, hist_index, ir);
+    return do_state_update(c, hist, hist_index, ir, false, next_q);
 }
 
+// One whole iteration (fpi_solver.f90:22-36).  Its state update also
+// computes the next iteration's U, q and time-step denominators, so an
+// iteration that follows another skips compute_conserved_vector and
+// q_variables (the same values, one pass earlier); every other change of
+// prim or U clears q_fresh.
 int do_iteration(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
     const DevGeom g = c->geom();
     const DevState s = c->state();
-    launch_conserved_and_q(g, s, c->fp, c->stream);
+    if (!c->q_fresh) launch_conserved_and_q(g, s, c->fp, c->stream);
     if (c->fp.inner_iterations > 0) {
         launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
         launch_timestep(g, s, c->fp, c->stream);
@@ -623,7 +634,7 @@ int do_iteration(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResid
     flux(c, kFluxAll);
     int rc = launch_error("iteration");
     if (rc) return rc;
-    return do_sweeps_and_update(c, hist, hist_index, ir);
+    return do_sweeps_and_update(c, hist, hist_index, ir, true);
 }
 
 // fpi_solver.f90:38-43
@@ -679,7 +690,7 @@ HostList host_list(const mfcfd_geometry* gm, int r, int d) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-const char* mfcfd_last_error(void) { return g_last_error.c_str(); }
+const char* mfcfd_last_error(void) { return g_last_error; }
 int mfcfd_abi_version(void) { return MFCFD_ABI_VERSION; }
 
 int mfcfd_create(int device, mfcfd_ctx** out) {
@@ -715,6 +726,7 @@ int mfcfd_destroy(mfcfd_ctx* ctx) {
 
 int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
     if (!c || !p) return fail(MFCFD_EINVAL, "null argument");
+    c->q_fresh = false;
     if (p->mode != MFCFD_EXPLICIT && p->mode != MFCFD_IMPLICIT && p->mode != MFCFD_IMPLICIT_BACKUP)
         return fail(MFCFD_EINVAL, "mode %d", p->mode);
     if (p->precision != MFCFD_FP64 && p->precision != MFCFD_MIXED)
@@ -824,6 +836,11 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
             if (!hl.p) continue;
             if (hl.count < 0 || hl.count > kConnCols)
                 return fail(MFCFD_EINVAL, "point %d: list %d has %d entries", i + 1, d, hl.count);
+            if (hl.count == 0)
+                return fail(MFCFD_EINVAL,
+                            "point %d: list %d, which its flux walks, is empty; the reference would divide "
+                            "by the zero determinant of its least-squares matrix",
+                            i + 1, d);
             if (hl.p != gm->conn + 27LL * i && 22LL * i + hl.count > 22LL * n)
                 return fail(MFCFD_EINVAL,
                             "point %d: split list %d has %d entries and runs past the (22,N) array "
@@ -1010,6 +1027,27 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     if (c->dgeo.alloc(12ULL * n) != hipSuccess) return fail(MFCFD_ENOMEM, "D_inv geometry");
     launch_dinv_geometry(c->geom(), c->dgeo.p, c->fp.power, st);
     if ((rc = launch_error("stencil coefficients"))) return rc;
+    {
+        // singular least-squares stencils: the reference divides by their
+        // zero determinant; refused here instead
+        if (c->stencil_bad.alloc(1) != hipSuccess) return fail(MFCFD_ENOMEM, "stencil check");
+        const int none = 0x7fffffff;
+        int bad = none;
+        HIP_TRY(hipMemcpyAsync(c->stencil_bad.p, &none, sizeof(int), hipMemcpyHostToDevice, st));
+        launch_stencil_check(c->geom(), c->fp.power, c->stencil_bad.p, st);
+        if ((rc = launch_error("stencil check"))) return rc;
+        HIP_TRY(hipMemcpyAsync(&bad, c->stencil_bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (bad != none) {
+            static const char* names[7] = {"xpos", "xneg", "ypos", "yneg", "zpos", "zneg", "conn"};
+            const int pt = perm[bad / 8] + 1;
+            c->release_all();
+            return fail(MFCFD_EINVAL,
+                        "point %d: the least-squares matrix of its %s stencil is singular (coplanar or collinear "
+                        "neighbours); the reference would divide by its zero determinant",
+                        pt, names[bad % 8]);
+        }
+    }
 
     // flow state (zero until uploaded; initial_conditions.f90:23-24 zero delUp/delUn)
     DevBuf<double>* st5[] = {&c->prim, &c->U, &c->q, &c->flux_res, &c->delUp, &c->delUn};
@@ -1132,6 +1170,7 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     HIP_TRY(hipStreamSynchronize(st));
     c->have_geom = true;
     c->graph_stale = true;
+    c->q_fresh = false;
     c->res_old = c->res_new = c->residue = 0.0;
     c->last_t = 0;
     return MFCFD_OK;
@@ -1282,6 +1321,21 @@ int mfcfd_upload(mfcfd_ctx* c, int id, const double* host) {
     ArrayDesc a = array_of(c, id);
     if (!a.buf || !host) return fail(MFCFD_EINVAL, "bad array id %d or null pointer", id);
     const size_t bytes = (size_t)a.per_point * c->n * sizeof(double);
+    if (id == MFCFD_ARR_PRIM) {
+        // a state the iteration cannot start from: q_variables.f90 takes the
+        // log of density and pressure, and NaN spreads through every stencil
+        for (long long r = 0; r < c->n; ++r) {
+            const double* p = host + 5 * r;
+            bool ok = p[0] > 0.0 && p[4] > 0.0;
+            for (int v = 0; v < 5; ++v) ok = ok && std::isfinite(p[v]);
+            if (!ok)
+                return fail(MFCFD_EINVAL,
+                            "prim of point %lld = (%g, %g, %g, %g, %g): not finite with positive density and "
+                            "pressure",
+                            r + 1, p[0], p[1], p[2], p[3], p[4]);
+        }
+    }
+    c->q_fresh = false;
     HIP_TRY(hipMemcpyAsync(c->staging.p, host, bytes, hipMemcpyHostToDevice, c->stream));
     launch_gather(a.buf->p, c->staging.p, c->perm.p, c->n, a.per_point, c->stream);
     // mixed form: the double array is staging for the FP32 copy the kernels read
@@ -1310,6 +1364,7 @@ int mfcfd_download(mfcfd_ctx* c, int id, double* host) {
 int mfcfd_compute_conserved_vector(mfcfd_ctx* c) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
+    c->q_fresh = false;
     launch_conserved_vector(c->geom(), c->state(), c->stream);
     return launch_error("compute_conserved_vector");
 }
@@ -1317,6 +1372,7 @@ int mfcfd_compute_conserved_vector(mfcfd_ctx* c) {
 int mfcfd_eval_q_variables(mfcfd_ctx* c) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
+    c->q_fresh = false;
     launch_q_variables(c->geom(), c->state(), c->fp, c->stream);
     return launch_error("eval_q_variables");
 }
@@ -1423,6 +1479,13 @@ int mfcfd_run_graph(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, dou
     ir.chunk_ptr = c->chunk_ptr.p;
     ir.n_chunks = c->n_chunks;
     ir.chunk_sum = c->chunk_sum.p;
+    // the captured iteration starts from the q of the previous one: the
+    // first replay after any other change of prim takes it from here
+    if (!c->q_fresh) {
+        launch_conserved_and_q(c->geom(), c->state(), c->fp, c->stream);
+        if ((rc = launch_error("conserved and q"))) return rc;
+        c->q_fresh = true;
+    }
     if (c->graph_stale || !c->graph_exec) {
         if (c->graph_exec) {
             (void)hipGraphExecDestroy(c->graph_exec);
@@ -1507,6 +1570,7 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
     if (repeats <= 0) repeats = 3;
+    c->q_fresh = false;
     const DevGeom g = c->geom();
     const DevState s = c->state();
     const int blocks[3] = {64, 128, 256};
@@ -1623,7 +1687,7 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
     // (the alternative driver's old entries read values of the previous
     // repeat, so its sweeps are not tuned)
     if (c->params.mode == MFCFD_IMPLICIT && c->params.sweep_schedule == MFCFD_SWEEP_ASYNC) {
-        const int divs[3] = {1, 2, 4};
+        const int divs[4] = {1, 2, 4, 8};
         double best_ms = 1e30;
         for (int dv : divs) {
             c->shapes.sweep_div = dv;

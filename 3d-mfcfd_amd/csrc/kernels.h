@@ -47,6 +47,10 @@ struct SweepSchedule {
     int n_swept;
 };
 
+// counters of the sync-free sweeps: [0] ticket, [1] finished workgroups
+// (the rest spare)
+constexpr int kSweepCounters = 8;
+
 // sync-free sweeps (kernels_sweep.hip); one per sweep direction, from the
 // host plan (SweepPlan in preprocess.h)
 struct SweepAsync {
@@ -68,13 +72,16 @@ struct SweepAsync {
     double* dsrc;               // [5 n_wait_src] published delU of the copy sources read as this
                                 // sweep's value (fine-grained; may be null)
     const int* src_slot;        // [n] slot of each point in dsrc, -1 if none (may be null)
-    unsigned* ctr;              // [2] ticket counter, finished workgroups (zero between sweeps)
+    unsigned* ctr;              // [kSweepCounters] ticket counter, finished workgroups (zero between sweeps)
     int* stalls;                // waits that hit the spin limit
     unsigned long long zero;    // 0: the operand of the fallback polling atomic adds
 };
 
 // setup: Cramer weights of every direction list (geometry only)
 void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, hipStream_t st);
+// setup: *first_bad = min(8*point + list) over singular least-squares stencils
+// (list 0..5 the direction lists, 6 conn); untouched when there are none
+void launch_stencil_check(const DevGeom& g, double power, int* first_bad, hipStream_t st);
 // boundary permutations: dst[k*m + c] = src[perm[k]*m + c] and the inverse
 void launch_gather(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st);
 // precision conversions of a flat array (mixed form uploads and downloads)
@@ -144,11 +151,13 @@ struct InstanceResidual {
 };
 // hist[hist_index] (or hist[*ir->counter] when ir is given) receives res_new;
 // with_explicit_sweep: do the explicit form's sweep (launch_explicit_sweep)
-// in the same pass
+// in the same pass; next_q: also compute the next iteration's U, q and
+// time-step denominators from the new prim (launch_conserved_and_q's work)
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p,
                          const TypeLists& tl, double* partial, int* partial_pos, int n_partial,
                          ResidualRecord* rec, double* hist, int hist_index, hipStream_t st,
-                         const InstanceResidual* ir = nullptr, bool with_explicit_sweep = false);
+                         const InstanceResidual* ir = nullptr, bool with_explicit_sweep = false,
+                         bool next_q = false);
 int state_update_blocks(int n);
 
 }  // namespace mfcfd

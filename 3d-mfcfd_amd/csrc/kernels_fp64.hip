@@ -56,9 +56,85 @@ __global__ void __launch_bounds__(kBlock) k_stencil_coef(DevGeom g, double* coef
     }
 }
 
+// Bind-time check of every least-squares system the iteration solves: the
+// direction lists the flux stencils walk and each point's conn, whose 3x3
+// weighted normal matrices the reference inverts by Cramer's rule
+// (interior_flux_dGxpos.f90:84-95, q_derivatives.f90:44-61).  A coplanar or
+// collinear stencil makes that matrix singular and the reference divides by
+// a zero determinant.  Flags the smallest (point, list) whose determinant is
+// not above 1e-12 of its scale ((trace/3)^3, invariant under the frame's
+// rotation), NaN included: 8*point + list, list 6 = conn.  Empty walked lists
+// are refused on the host.
+template <bool UW>
+__global__ void __launch_bounds__(kBlock) k_stencil_check(DevGeom g, double power, int* first_bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const Frame<double> f = load_frame(g.frame, i);
+    const double4 pi4 = g.pos[i];
+    for (int d = 0; d <= 6; ++d) {
+        const int cnt = d < 6 ? g.dir_cnt[(long long)d * g.n + i] : g.nbhs[i];
+        const int* list = d < 6 ? g.lst + g.dir_off[(long long)d * g.n + i] : g.conn + kConnCols * (long long)i;
+        if (cnt == 0) continue;
+        Geo6<double> geo = {0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < cnt; ++j) {
+            const double4 pk = g.pos[list[j]];
+            const double delx = pk.x - pi4.x, dely = pk.y - pi4.y, delz = pk.z - pi4.z;
+            const double dels = delx * f.t1[0] + dely * f.t1[1] + delz * f.t1[2];
+            const double delt = delx * f.t2[0] + dely * f.t2[1] + delz * f.t2[2];
+            const double deln = delx * f.n[0] + dely * f.n[1] + delz * f.n[2];
+            const double w = wls_weight<UW>(dels, delt, deln, power);
+            geo.xx += dels * dels * w;
+            geo.yy += delt * delt * w;
+            geo.zz += deln * deln * w;
+            geo.xy += dels * delt * w;
+            geo.yz += delt * deln * w;
+            geo.zx += deln * dels * w;
+        }
+        const double tr = (geo.xx + geo.yy + geo.zz) / 3.0;
+        const double det = cramer_det(geo);
+        if (!(fabs(det) > 1e-12 * tr * tr * tr)) atomicMin(first_bad, 8 * i + d);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // compute_conserved_vector.f90:20-32 and q_variables.f90:18-38
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void conserved_point(const DevState& s, long long k, const double p[5],
+                                                const Frame<double>& f) {
+    double U[5];
+    U[0] = p[0];
+    double temp1 = p[0] * p[1];
+    const double temp2 = p[0] * p[2];
+    const double temp3 = p[0] * p[3];
+    U[1] = temp1 * f.t1[0] + temp2 * f.t1[1] + temp3 * f.t1[2];
+    U[2] = temp1 * f.t2[0] + temp2 * f.t2[1] + temp3 * f.t2[2];
+    U[3] = temp1 * f.n[0] + temp2 * f.n[1] + temp3 * f.n[2];
+    temp1 = p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
+    U[4] = 2.5 * p[4] + 0.5 * U[0] * temp1;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) s.U[5 * k + r] = U[r];
+}
+
+template <typename QT>
+__device__ __forceinline__ void q_point(const DevState& s, long long k, const double p[5]) {
+    const double rho = p[0], u1 = p[1], u2 = p[2], u3 = p[3], pr = p[4];
+    const double beta = 0.5 * rho / pr;
+    double q[5];
+    q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2 + u3 * u3);
+    const double two_times_beta = 2.0 * beta;
+    q[1] = two_times_beta * u1;
+    q[2] = two_times_beta * u2;
+    q[3] = two_times_beta * u3;
+    q[4] = -two_times_beta;
+    QT* qo = qarrays<QT>(s).qp(k);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) qo[r] = QT(q[r]);
+    // timestep_delt.f90:40-42: the neighbour-only part of the local step
+    const double den = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3) + 3.0 * __builtin_sqrt(pr / rho);
+    s.tsden[k] = den;
+    if (sizeof(QT) == 4) qo[5] = QT(den);
+}
+
 template <bool DO_U, bool DO_Q, typename QT>
 __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,39 +142,8 @@ __global__ void __launch_bounds__(kBlock) k_conserved_q(DevGeom g, DevState s) {
     double p[5];
 #pragma unroll
     for (int r = 0; r < 5; ++r) p[r] = s.prim[5 * (long long)k + r];
-    if (DO_U) {
-        const Frame<double> f = load_frame(g.frame, k);
-        double U[5];
-        U[0] = p[0];
-        double temp1 = p[0] * p[1];
-        const double temp2 = p[0] * p[2];
-        const double temp3 = p[0] * p[3];
-        U[1] = temp1 * f.t1[0] + temp2 * f.t1[1] + temp3 * f.t1[2];
-        U[2] = temp1 * f.t2[0] + temp2 * f.t2[1] + temp3 * f.t2[2];
-        U[3] = temp1 * f.n[0] + temp2 * f.n[1] + temp3 * f.n[2];
-        temp1 = p[1] * p[1] + p[2] * p[2] + p[3] * p[3];
-        U[4] = 2.5 * p[4] + 0.5 * U[0] * temp1;
-#pragma unroll
-        for (int r = 0; r < 5; ++r) s.U[5 * (long long)k + r] = U[r];
-    }
-    if (DO_Q) {
-        const double rho = p[0], u1 = p[1], u2 = p[2], u3 = p[3], pr = p[4];
-        const double beta = 0.5 * rho / pr;
-        double q[5];
-        q[0] = log(rho) + (log(beta) * 2.5) - beta * (u1 * u1 + u2 * u2 + u3 * u3);
-        const double two_times_beta = 2.0 * beta;
-        q[1] = two_times_beta * u1;
-        q[2] = two_times_beta * u2;
-        q[3] = two_times_beta * u3;
-        q[4] = -two_times_beta;
-        QT* qo = qarrays<QT>(s).qp(k);
-#pragma unroll
-        for (int r = 0; r < 5; ++r) qo[r] = QT(q[r]);
-        // timestep_delt.f90:40-42: the neighbour-only part of the local step
-        const double den = __builtin_sqrt(u1 * u1 + u2 * u2 + u3 * u3) + 3.0 * __builtin_sqrt(pr / rho);
-        s.tsden[k] = den;
-        if (sizeof(QT) == 4) qo[5] = QT(den);
-    }
+    if (DO_U) conserved_point(s, k, p, load_frame(g.frame, k));
+    if (DO_Q) q_point<QT>(s, k, p);
 }
 
 // the time step's denominators alone (time step called outside an iteration)
@@ -766,7 +811,12 @@ __device__ __forceinline__ MaxPair max_pair(MaxPair a, MaxPair b) {
 // BACKUP: state_update_implicit.f90:22-60, paired with the alternative
 // sweep driver: wall points keep U(4), nothing is copied into delUp here,
 // outer points take Ubar + delUn
-template <bool EXPLICIT, bool BACKUP = false>
+// NEXTQ (1: FP64 q, 2: the mixed form's FP32 q): also the next iteration's
+// compute_conserved_vector and q_variables from the new prim, the same
+// arithmetic one pass earlier (supersonic points: in k_supersonic, after
+// their copy); U then holds the conserved vector of the new prim, as it does
+// once the next iteration has begun
+template <bool EXPLICIT, bool BACKUP = false, int NEXTQ = 0>
 __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, FlowParams fp,
                                                          double* partial, int* partial_pos,
                                                          int* partial_pt, double* partial_max,
@@ -813,6 +863,11 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
                 s.prim[5 * (long long)k + r] = prim[r];
                 if (!EXPLICIT && !BACKUP) s.delUp[5 * (long long)k + r] = dU[r];
             }
+            if (NEXTQ) {
+                conserved_point(s, k, prim, f);
+                if (NEXTQ == 1) q_point<double>(s, k, prim);
+                else q_point<float>(s, k, prim);
+            }
         } else if (st == 2) {
             const Frame<double> f = load_frame(g.frame, k);
             double U[5], prim[5];
@@ -829,6 +884,11 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
             conserved_to_prim(U, f, prim);
 #pragma unroll
             for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = prim[r];
+            if (NEXTQ) {
+                conserved_point(s, k, prim, f);
+                if (NEXTQ == 1) q_point<double>(s, k, prim);
+                else q_point<float>(s, k, prim);
+            }
         }
     }
     if (ressq && k < g.n) ressq[k] = res_sqr;
@@ -867,8 +927,9 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
 // state_update.f90:80-114: supersonic points copy the nearest neighbour's
 // state after every other point is updated (outlet list, then inlet list,
 // each in order, so a single thread walks them)
+// next_q as the state update's NEXTQ, for the copied points
 __global__ void k_supersonic(DevGeom g, DevState s, const int* out_idx, int n_out, const int* in_idx,
-                             int n_in, int skip_wall) {
+                             int n_in, int skip_wall, int next_q) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int pass = 0; pass < 2; ++pass) {
         const int* idx = pass == 0 ? out_idx : in_idx;
@@ -876,7 +937,16 @@ __global__ void k_supersonic(DevGeom g, DevState s, const int* out_idx, int n_ou
         for (int m = 0; m < cnt; ++m) {
             const int k = idx[m];
             const int p = nearest_neighbour_dev(g, k, 100000.0, skip_wall != 0);
-            for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = s.prim[5 * (long long)p + r];
+            double prim[5];
+            for (int r = 0; r < 5; ++r) {
+                prim[r] = s.prim[5 * (long long)p + r];
+                s.prim[5 * (long long)k + r] = prim[r];
+            }
+            if (next_q) {
+                conserved_point(s, k, prim, load_frame(g.frame, k));
+                if (next_q == 1) q_point<double>(s, k, prim);
+                else q_point<float>(s, k, prim);
+            }
         }
     }
 }
@@ -986,6 +1056,15 @@ void launch_stencil_coefficients(const DevGeom& g, double* coef, double power, h
     else
         hipLaunchKernelGGL(k_stencil_coef<false>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, coef,
                            power);
+}
+void launch_stencil_check(const DevGeom& g, double power, int* first_bad, hipStream_t st) {
+    if (g.n <= 0) return;
+    if (power == 0.0)
+        hipLaunchKernelGGL(k_stencil_check<true>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, power,
+                           first_bad);
+    else
+        hipLaunchKernelGGL(k_stencil_check<false>, dim3(grid_for(g.n, kBlock)), dim3(kBlock), 0, st, g, power,
+                           first_bad);
 }
 void launch_gather(double* dst, const double* src, const int* perm, int n, int m, hipStream_t st) {
     if (n <= 0) return;
@@ -1157,25 +1236,43 @@ void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams&
                            (const double*)s.delUn, 5LL * g.n);
 }
 int state_update_blocks(int n) { return grid_for(n, kBlock); }
+template <bool EXPLICIT, bool BACKUP>
+void launch_state_update_kernel(int next_q, const DevGeom& g, const DevState& s, const FlowParams& p,
+                                double* partial, int* partial_pos, int* partial_pt, double* partial_max,
+                                double* ressq, int n_partial, hipStream_t st) {
+    if (next_q == 1)
+        hipLaunchKernelGGL((k_state_update<EXPLICIT, BACKUP, 1>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p,
+                           partial, partial_pos, partial_pt, partial_max, ressq);
+    else if (next_q == 2)
+        hipLaunchKernelGGL((k_state_update<EXPLICIT, BACKUP, 2>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p,
+                           partial, partial_pos, partial_pt, partial_max, ressq);
+    else
+        hipLaunchKernelGGL((k_state_update<EXPLICIT, BACKUP, 0>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p,
+                           partial, partial_pos, partial_pt, partial_max, ressq);
+}
+
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                          double* partial, int* partial_pos, int n_partial, ResidualRecord* rec, double* hist,
-                         int hist_index, hipStream_t st, const InstanceResidual* ir, bool with_explicit_sweep) {
+                         int hist_index, hipStream_t st, const InstanceResidual* ir, bool with_explicit_sweep,
+                         bool next_q) {
     // partial buffer layout: [sum | max] doubles, [pos | point] ints
     double* partial_max = partial + n_partial;
     int* partial_pt = partial_pos + n_partial;
     const bool backup = p.backup && !with_explicit_sweep;
+    const int nq = next_q ? (p.mixed ? 2 : 1) : 0;
+    double* ressq = ir ? ir->ressq : nullptr;
     if (with_explicit_sweep)
-        hipLaunchKernelGGL((k_state_update<true, false>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
-                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
+        launch_state_update_kernel<true, false>(nq, g, s, p, partial, partial_pos, partial_pt, partial_max, ressq,
+                                                n_partial, st);
     else if (backup)
-        hipLaunchKernelGGL((k_state_update<false, true>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
-                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
+        launch_state_update_kernel<false, true>(nq, g, s, p, partial, partial_pos, partial_pt, partial_max, ressq,
+                                                n_partial, st);
     else
-        hipLaunchKernelGGL((k_state_update<false, false>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
-                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
+        launch_state_update_kernel<false, false>(nq, g, s, p, partial, partial_pos, partial_pt, partial_max, ressq,
+                                                 n_partial, st);
     if (tl.n_sup_in + tl.n_sup_out > 0)
         hipLaunchKernelGGL(k_supersonic, dim3(1), dim3(1), 0, st, g, s, tl.sup_out, tl.n_sup_out, tl.sup_in,
-                           tl.n_sup_in, backup ? 1 : 0);
+                           tl.n_sup_in, backup ? 1 : 0, nq);
     // state_update_implicit.f90:98-100
     if (backup)
         hipLaunchKernelGGL(k_copy, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,

@@ -33,10 +33,12 @@
 //                  round thread 0 takes kSweepWaves consecutive positions
 //                  with one atomic and wave w works on position base + w.
 //                  Lane e of the wave takes entry e (e + 64, ... for longer
-//                  lists): Gold from the point's prim before the sweep, then
-//                  the neighbour's published primitive state -- polled until
-//                  it is no longer pending for a wait entry, read from pold
-//                  for an old one -- and Gnew from it.  The wave's sum is a
+//                  lists): Gold from the neighbour's state before the sweep
+//                  first (it does not depend on the sweep, so its loads and
+//                  arithmetic overlap the wait), then the neighbour's
+//                  published primitive state -- polled until it is no
+//                  longer pending for a wait entry, read from pold for an
+//                  old one -- and Gnew from it.  The wave's sum is a
 //                  fixed-order DPP reduction.  Lane 0 stores the point's delU
 //                  and its published state prim(U_i + delU_i), computed once
 //                  here instead of once per entry that reads it.
@@ -102,22 +104,32 @@ __device__ __forceinline__ void store_value(double* p, double v) {
                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// read five values written in this sweep, waiting until none is pending:
-// plain agent-scope loads first (the value is usually there already), then
-// device-scope atomic adds of zero until every value has arrived.  Gives up
-// after the spin limit, or at once when another wait of this sweep already
-// gave up, so a broken schedule ends the kernel instead of stalling it point
-// by point.
+// read five values written in this sweep, waiting until none is pending.
+// One agent-scope load of each first (the value is usually there already);
+// while some are pending, only the last pending one is polled -- the
+// producer stores the five in order, so when it arrives the others have
+// mostly arrived too -- and then every pending one is read again.  After
+// kSpinAtomic polls the reads become device-scope atomic adds of zero,
+// performed at the device's coherence point.  A value that is not pending
+// is final.  Gives up after the spin limit, or at once when another wait of
+// this sweep already gave up, so a broken schedule ends the kernel instead
+// of stalling it point by point.
+__device__ __forceinline__ unsigned long long poll_value(unsigned long long* p, int spins,
+                                                         unsigned long long zero) {
+    return spins < kSpinAtomic ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : __hip_atomic_fetch_add(p, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void wait_read(double* v5, double out[5], int* stalls, unsigned long long zero) {
     unsigned long long* w = reinterpret_cast<unsigned long long*>(v5);
-    bool ok = true;
+    unsigned pending = 0;
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
         const unsigned long long b = __hip_atomic_load(w + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         out[r] = __longlong_as_double((long long)b);
-        ok = ok && b != kPending;
+        if (b == kPending) pending |= 1u << r;
     }
-    if (ok) return;
+    if (!pending) return;
     int spins = 0;
 #pragma clang loop unroll(disable)
     for (;;) {
@@ -129,17 +141,17 @@ __device__ __forceinline__ void wait_read(double* v5, double out[5], int* stalls
             return;
         }
         __builtin_amdgcn_s_sleep(1);
-        ok = true;
+        const int last = 31 - __builtin_clz(pending);
+        if (poll_value(w + last, spins, zero) == kPending) continue;
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            const unsigned long long b =
-                spins < kSpinAtomic
-                    ? __hip_atomic_load(w + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                    : __hip_atomic_fetch_add(w + r, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            out[r] = __longlong_as_double((long long)b);
-            ok = ok && b != kPending;
+            if (pending & (1u << r)) {
+                const unsigned long long b = poll_value(w + r, spins, zero);
+                out[r] = __longlong_as_double((long long)b);
+                if (b != kPending) pending &= ~(1u << r);
+            }
         }
-        if (ok) return;
+        if (!pending) return;
     }
 }
 
@@ -326,6 +338,10 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 // the neighbour's state of this sweep first: nothing is held
                 // across the wait (a point's dependencies have mostly
                 // finished before its wave reaches them), then both fluxes
+                T prim_old[5], Gold[5];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k + r]);
+                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
                 double pk[5];
                 if (packed & kTakeWait) {
                     wait_read(a.pub + 5 * k, pk, a.stalls, a.zero);
@@ -337,10 +353,6 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
 #pragma unroll
                 for (int r = 0; r < 5; ++r) pt[r] = T(pk[r]);
                 flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
-                T prim_old[5], Gold[5];
-#pragma unroll
-                for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k + r]);
-                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
 #pragma unroll
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }
@@ -418,9 +430,9 @@ void launch_one(const DevGeom& g, const DevState& s, const FlowParams& p, const 
                            (const double*)dU, a);
     const long long n_mark = 5LL * ((long long)a.count + a.n_wait_src);
     hipLaunchKernelGGL(k_sweep_mark, dim3(grid_for(n_mark, kBlock)), dim3(kBlock), 0, st, a);
-    if (ev) hipEventRecord(ev[0], st);
+    if (ev) (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL((k_sweep_async<FORWARD, UW, T>), dim3(grid), dim3(kSweepBlock), 0, st, g, s, p, a);
-    if (ev) hipEventRecord(ev[1], st);
+    if (ev) (void)hipEventRecord(ev[1], st);
 }
 
 template <bool UW, typename T>

@@ -201,6 +201,10 @@ int build_dir(const SweepPlanInputs& in, bool forward, SweepDirPlan& out) {
         exec_at[visit[u]] = e;
         out.take.insert(out.take.end(), vtake.begin() + vptr[u], vtake.begin() + vptr[u + 1]);
         out.take_ptr[e + 1] = (long long)out.take.size();
+        // by neighbour: a neighbour sits in up to one list per axis (more
+        // with the spill), and its entries then sit on adjacent lanes, of
+        // which the first reads the neighbour's state for all of them
+        std::sort(out.take.begin() + out.take_ptr[e], out.take.end());
     }
     // every wait entry must read a point that executes earlier
     for (int e = 0; e < m; ++e)

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MFCFD_ABI_VERSION 5
+#define MFCFD_ABI_VERSION 6
 
 /* error codes */
 #define MFCFD_OK 0
@@ -136,7 +136,11 @@ int mfcfd_set_params(mfcfd_ctx *ctx, const mfcfd_params *p);
 /* copies the point arrays to HBM once; replaces the host->module-global
  * handoff after read_input_point_data (point_preprocessor.f90:13-98),
  * generate_split_stencils (generate_connectivity.f90:12-34) and aliasing
- * (implicit_aliasing.f90:13-44).  Also builds the sweep schedule. */
+ * (implicit_aliasing.f90:13-44).  Also builds the sweep schedule.
+ * Returns MFCFD_EINVAL, naming the point, for an empty walked direction list
+ * or a least-squares stencil (direction list or conn) whose 3x3 matrix is
+ * singular -- coplanar or collinear neighbours, where the reference divides
+ * by a zero determinant. */
 int mfcfd_bind_geometry(mfcfd_ctx *ctx, const mfcfd_geometry *g);
 
 /* the same binding with one argument per array, for hosts that pass arrays
@@ -174,7 +178,10 @@ int mfcfd_bind_batch(mfcfd_ctx *ctx, int n_instances, const mfcfd_geometry *geom
 /* number of clouds bound (1 for mfcfd_bind_geometry) and their point counts */
 int mfcfd_batch_info(mfcfd_ctx *ctx, int *n_instances, int *counts);
 
-/* state transfer by array id (MFCFD_ARR_*), reference layout */
+/* state transfer by array id (MFCFD_ARR_*), reference layout.  An upload of
+ * prim with a non-finite value, or a density or pressure not above zero,
+ * returns MFCFD_EINVAL and leaves the device state unchanged (the reference
+ * would carry the NaN into every stencil it touches). */
 int mfcfd_upload(mfcfd_ctx *ctx, int array_id, const double *host);
 int mfcfd_download(mfcfd_ctx *ctx, int array_id, double *host);
 

@@ -20,7 +20,8 @@ import tempfile
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liborc.so")
+# ORC_LIB selects another build of the same sources (the sanitizer build)
+LIB_PATH = os.environ.get("ORC_LIB") or os.path.join(HERE, "liborc.so")
 SPLIT_COLS = 22
 SPLIT_SLACK = 32
 

@@ -57,3 +57,33 @@ def fixture_params(d):
 @pytest.fixture(scope="session")
 def pkg():
     return PKG
+
+
+def degenerate_cloud(kind):
+    """The explicit_blocks fixture cloud with one interior point's xpos
+    stencil made degenerate: kind "collinear" moves each of its xpos
+    neighbours onto the point's tan1 axis at its own distance along it (the
+    list's least-squares matrix becomes rank one), kind "empty" mirrors them
+    to the negative side (the list empties).  The point is chosen so that its
+    other five lists keep entries.  Returns (cloud, point index 0-based)."""
+    pc = PKG.pointcloud
+    solver = PKG.solver
+    cl, _ = load_fixture("explicit_blocks")
+    cnt, lists = solver.split_stencils(cl)
+    p3 = np.stack([cl.x, cl.y, cl.z], axis=1)
+    for p in np.nonzero(cl.status == pc.STATUS_INTERIOR)[0]:
+        if cnt[0, p] < 3:
+            continue
+        ks = lists[0, 22 * p:22 * p + cnt[0, p]] - 1
+        if np.any(cl.status[ks] != pc.STATUS_INTERIOR):
+            continue
+        q = p3.copy()
+        dels = (p3[ks] - p3[p]) @ cl.tan1[p]
+        q[ks] = p3[p] + (dels if kind == "collinear" else -dels)[:, None] * cl.tan1[p][None, :]
+        out = pc.PointCloud(q[:, 0].copy(), q[:, 1].copy(), q[:, 2].copy(), cl.tan1, cl.tan2, cl.nor,
+                            cl.status, cl.nbhs, cl.conn, cl.min_dist)
+        c2, _ = solver.split_stencils(out)
+        want_empty = [0] if kind == "empty" else []
+        if all((c2[d, p] == 0) == (d in want_empty) for d in range(6)):
+            return out, int(p)
+    raise RuntimeError(f"no point of the fixture takes a {kind} xpos stencil")

@@ -206,9 +206,6 @@ struct mfcfd_ctx {
     // one iteration captured as a HIP graph (mfcfd_run_graph)
     hipGraphExec_t graph_exec = nullptr;
     bool graph_stale = true;
-    // U, q (q32) and tsden hold compute_conserved_vector and q_variables of
-    // the current prim, computed by the last iteration's state update
-    bool q_fresh = false;
     int sweep_grid = 0, sweep_grid_mixed = 0;
     // bookkeeping of fpi_solver.f90:38-43
     double res_old = 0.0, res_new = 0.0, residue = 0.0, max_res = 0.0, sum_res_sqr = 0.0;
@@ -596,35 +593,26 @@ int prepare_plan(mfcfd_ctx* c) {
     return build_plan(c, c->params.mode);
 }
 
-// next_q: the pass also computes the next iteration's U, q and time-step
-// denominators (an iteration that finds q_fresh skips its first step)
 int do_state_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr,
-                    bool with_explicit_sweep = false, bool next_q = false) {
+                    bool with_explicit_sweep = false) {
     launch_state_update(c->geom(), c->state(), c->fp, c->tl, c->partial.p, c->partial_pos.p, c->n_partial,
-                        c->rec.p, hist, hist_index, c->stream, ir, with_explicit_sweep, next_q);
-    c->q_fresh = next_q;
+                        c->rec.p, hist, hist_index, c->stream, ir, with_explicit_sweep);
     return launch_error("state update");
 }
 
 // the sweeps and the state update of one iteration; the explicit form's
 // sweep runs inside the state update's pass
-int do_sweeps_and_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr,
-                         bool next_q = false) {
-    if (c->params.mode == MFCFD_EXPLICIT) return do_state_update(c, hist, hist_index, ir, true, next_q);
+int do_sweeps_and_update(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
+    if (c->params.mode == MFCFD_EXPLICIT) return do_state_update(c, hist, hist_index, ir, true);
     int rc = do_sweeps(c);
     if (rc) return rc;
-    return do_state_update(c, hist, hist_index, ir, false, next_q);
+    return do_state_update(c, hist, hist_index, ir);
 }
 
-// One whole iteration (fpi_solver.f90:22-36).  Its state update also
-// computes the next iteration's U, q and time-step denominators, so an
-// iteration that follows another skips compute_conserved_vector and
-// q_variables (the same values, one pass earlier); every other change of
-// prim or U clears q_fresh.
 int do_iteration(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResidual* ir = nullptr) {
     const DevGeom g = c->geom();
     const DevState s = c->state();
-    if (!c->q_fresh) launch_conserved_and_q(g, s, c->fp, c->stream);
+    launch_conserved_and_q(g, s, c->fp, c->stream);
     if (c->fp.inner_iterations > 0) {
         launch_q_derivatives(g, s, c->fp, c->temp.p, c->stream);
         launch_timestep(g, s, c->fp, c->stream);
@@ -634,7 +622,7 @@ int do_iteration(mfcfd_ctx* c, double* hist, int hist_index, const InstanceResid
     flux(c, kFluxAll);
     int rc = launch_error("iteration");
     if (rc) return rc;
-    return do_sweeps_and_update(c, hist, hist_index, ir, true);
+    return do_sweeps_and_update(c, hist, hist_index, ir);
 }
 
 // fpi_solver.f90:38-43
@@ -726,7 +714,6 @@ int mfcfd_destroy(mfcfd_ctx* ctx) {
 
 int mfcfd_set_params(mfcfd_ctx* c, const mfcfd_params* p) {
     if (!c || !p) return fail(MFCFD_EINVAL, "null argument");
-    c->q_fresh = false;
     if (p->mode != MFCFD_EXPLICIT && p->mode != MFCFD_IMPLICIT && p->mode != MFCFD_IMPLICIT_BACKUP)
         return fail(MFCFD_EINVAL, "mode %d", p->mode);
     if (p->precision != MFCFD_FP64 && p->precision != MFCFD_MIXED)
@@ -1170,7 +1157,6 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
     HIP_TRY(hipStreamSynchronize(st));
     c->have_geom = true;
     c->graph_stale = true;
-    c->q_fresh = false;
     c->res_old = c->res_new = c->residue = 0.0;
     c->last_t = 0;
     return MFCFD_OK;
@@ -1335,7 +1321,6 @@ int mfcfd_upload(mfcfd_ctx* c, int id, const double* host) {
                             r + 1, p[0], p[1], p[2], p[3], p[4]);
         }
     }
-    c->q_fresh = false;
     HIP_TRY(hipMemcpyAsync(c->staging.p, host, bytes, hipMemcpyHostToDevice, c->stream));
     launch_gather(a.buf->p, c->staging.p, c->perm.p, c->n, a.per_point, c->stream);
     // mixed form: the double array is staging for the FP32 copy the kernels read
@@ -1364,7 +1349,6 @@ int mfcfd_download(mfcfd_ctx* c, int id, double* host) {
 int mfcfd_compute_conserved_vector(mfcfd_ctx* c) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
-    c->q_fresh = false;
     launch_conserved_vector(c->geom(), c->state(), c->stream);
     return launch_error("compute_conserved_vector");
 }
@@ -1372,7 +1356,6 @@ int mfcfd_compute_conserved_vector(mfcfd_ctx* c) {
 int mfcfd_eval_q_variables(mfcfd_ctx* c) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
-    c->q_fresh = false;
     launch_q_variables(c->geom(), c->state(), c->fp, c->stream);
     return launch_error("eval_q_variables");
 }
@@ -1479,13 +1462,6 @@ int mfcfd_run_graph(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, dou
     ir.chunk_ptr = c->chunk_ptr.p;
     ir.n_chunks = c->n_chunks;
     ir.chunk_sum = c->chunk_sum.p;
-    // the captured iteration starts from the q of the previous one: the
-    // first replay after any other change of prim takes it from here
-    if (!c->q_fresh) {
-        launch_conserved_and_q(c->geom(), c->state(), c->fp, c->stream);
-        if ((rc = launch_error("conserved and q"))) return rc;
-        c->q_fresh = true;
-    }
     if (c->graph_stale || !c->graph_exec) {
         if (c->graph_exec) {
             (void)hipGraphExecDestroy(c->graph_exec);
@@ -1570,7 +1546,6 @@ int mfcfd_autotune(mfcfd_ctx* c, int repeats, mfcfd_tuning* out) {
     int rc = check_ctx(c, true);
     if (rc) return rc;
     if (repeats <= 0) repeats = 3;
-    c->q_fresh = false;
     const DevGeom g = c->geom();
     const DevState s = c->state();
     const int blocks[3] = {64, 128, 256};

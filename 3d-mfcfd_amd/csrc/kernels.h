@@ -151,13 +151,11 @@ struct InstanceResidual {
 };
 // hist[hist_index] (or hist[*ir->counter] when ir is given) receives res_new;
 // with_explicit_sweep: do the explicit form's sweep (launch_explicit_sweep)
-// in the same pass; next_q: also compute the next iteration's U, q and
-// time-step denominators from the new prim (launch_conserved_and_q's work)
+// in the same pass
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p,
                          const TypeLists& tl, double* partial, int* partial_pos, int n_partial,
                          ResidualRecord* rec, double* hist, int hist_index, hipStream_t st,
-                         const InstanceResidual* ir = nullptr, bool with_explicit_sweep = false,
-                         bool next_q = false);
+                         const InstanceResidual* ir = nullptr, bool with_explicit_sweep = false);
 int state_update_blocks(int n);
 
 }  // namespace mfcfd

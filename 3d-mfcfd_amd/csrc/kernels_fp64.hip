@@ -811,12 +811,7 @@ __device__ __forceinline__ MaxPair max_pair(MaxPair a, MaxPair b) {
 // BACKUP: state_update_implicit.f90:22-60, paired with the alternative
 // sweep driver: wall points keep U(4), nothing is copied into delUp here,
 // outer points take Ubar + delUn
-// NEXTQ (1: FP64 q, 2: the mixed form's FP32 q): also the next iteration's
-// compute_conserved_vector and q_variables from the new prim, the same
-// arithmetic one pass earlier (supersonic points: in k_supersonic, after
-// their copy); U then holds the conserved vector of the new prim, as it does
-// once the next iteration has begun
-template <bool EXPLICIT, bool BACKUP = false, int NEXTQ = 0>
+template <bool EXPLICIT, bool BACKUP = false>
 __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, FlowParams fp,
                                                          double* partial, int* partial_pos,
                                                          int* partial_pt, double* partial_max,
@@ -863,11 +858,6 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
                 s.prim[5 * (long long)k + r] = prim[r];
                 if (!EXPLICIT && !BACKUP) s.delUp[5 * (long long)k + r] = dU[r];
             }
-            if (NEXTQ) {
-                conserved_point(s, k, prim, f);
-                if (NEXTQ == 1) q_point<double>(s, k, prim);
-                else q_point<float>(s, k, prim);
-            }
         } else if (st == 2) {
             const Frame<double> f = load_frame(g.frame, k);
             double U[5], prim[5];
@@ -884,11 +874,6 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
             conserved_to_prim(U, f, prim);
 #pragma unroll
             for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = prim[r];
-            if (NEXTQ) {
-                conserved_point(s, k, prim, f);
-                if (NEXTQ == 1) q_point<double>(s, k, prim);
-                else q_point<float>(s, k, prim);
-            }
         }
     }
     if (ressq && k < g.n) ressq[k] = res_sqr;
@@ -927,9 +912,8 @@ __global__ void __launch_bounds__(kBlock) k_state_update(DevGeom g, DevState s, 
 // state_update.f90:80-114: supersonic points copy the nearest neighbour's
 // state after every other point is updated (outlet list, then inlet list,
 // each in order, so a single thread walks them)
-// next_q as the state update's NEXTQ, for the copied points
 __global__ void k_supersonic(DevGeom g, DevState s, const int* out_idx, int n_out, const int* in_idx,
-                             int n_in, int skip_wall, int next_q) {
+                             int n_in, int skip_wall) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int pass = 0; pass < 2; ++pass) {
         const int* idx = pass == 0 ? out_idx : in_idx;
@@ -937,16 +921,7 @@ __global__ void k_supersonic(DevGeom g, DevState s, const int* out_idx, int n_ou
         for (int m = 0; m < cnt; ++m) {
             const int k = idx[m];
             const int p = nearest_neighbour_dev(g, k, 100000.0, skip_wall != 0);
-            double prim[5];
-            for (int r = 0; r < 5; ++r) {
-                prim[r] = s.prim[5 * (long long)p + r];
-                s.prim[5 * (long long)k + r] = prim[r];
-            }
-            if (next_q) {
-                conserved_point(s, k, prim, load_frame(g.frame, k));
-                if (next_q == 1) q_point<double>(s, k, prim);
-                else q_point<float>(s, k, prim);
-            }
+            for (int r = 0; r < 5; ++r) s.prim[5 * (long long)k + r] = s.prim[5 * (long long)p + r];
         }
     }
 }
@@ -1236,43 +1211,25 @@ void launch_sweeps_levels(const DevGeom& g, const DevState& s, const FlowParams&
                            (const double*)s.delUn, 5LL * g.n);
 }
 int state_update_blocks(int n) { return grid_for(n, kBlock); }
-template <bool EXPLICIT, bool BACKUP>
-void launch_state_update_kernel(int next_q, const DevGeom& g, const DevState& s, const FlowParams& p,
-                                double* partial, int* partial_pos, int* partial_pt, double* partial_max,
-                                double* ressq, int n_partial, hipStream_t st) {
-    if (next_q == 1)
-        hipLaunchKernelGGL((k_state_update<EXPLICIT, BACKUP, 1>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p,
-                           partial, partial_pos, partial_pt, partial_max, ressq);
-    else if (next_q == 2)
-        hipLaunchKernelGGL((k_state_update<EXPLICIT, BACKUP, 2>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p,
-                           partial, partial_pos, partial_pt, partial_max, ressq);
-    else
-        hipLaunchKernelGGL((k_state_update<EXPLICIT, BACKUP, 0>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p,
-                           partial, partial_pos, partial_pt, partial_max, ressq);
-}
-
 void launch_state_update(const DevGeom& g, const DevState& s, const FlowParams& p, const TypeLists& tl,
                          double* partial, int* partial_pos, int n_partial, ResidualRecord* rec, double* hist,
-                         int hist_index, hipStream_t st, const InstanceResidual* ir, bool with_explicit_sweep,
-                         bool next_q) {
+                         int hist_index, hipStream_t st, const InstanceResidual* ir, bool with_explicit_sweep) {
     // partial buffer layout: [sum | max] doubles, [pos | point] ints
     double* partial_max = partial + n_partial;
     int* partial_pt = partial_pos + n_partial;
     const bool backup = p.backup && !with_explicit_sweep;
-    const int nq = next_q ? (p.mixed ? 2 : 1) : 0;
-    double* ressq = ir ? ir->ressq : nullptr;
     if (with_explicit_sweep)
-        launch_state_update_kernel<true, false>(nq, g, s, p, partial, partial_pos, partial_pt, partial_max, ressq,
-                                                n_partial, st);
+        hipLaunchKernelGGL((k_state_update<true, false>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
+                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
     else if (backup)
-        launch_state_update_kernel<false, true>(nq, g, s, p, partial, partial_pos, partial_pt, partial_max, ressq,
-                                                n_partial, st);
+        hipLaunchKernelGGL((k_state_update<false, true>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
+                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
     else
-        launch_state_update_kernel<false, false>(nq, g, s, p, partial, partial_pos, partial_pt, partial_max, ressq,
-                                                 n_partial, st);
+        hipLaunchKernelGGL((k_state_update<false, false>), dim3(n_partial), dim3(kBlock), 0, st, g, s, p, partial,
+                           partial_pos, partial_pt, partial_max, ir ? ir->ressq : nullptr);
     if (tl.n_sup_in + tl.n_sup_out > 0)
         hipLaunchKernelGGL(k_supersonic, dim3(1), dim3(1), 0, st, g, s, tl.sup_out, tl.n_sup_out, tl.sup_in,
-                           tl.n_sup_in, backup ? 1 : 0, nq);
+                           tl.n_sup_in, backup ? 1 : 0);
     // state_update_implicit.f90:98-100
     if (backup)
         hipLaunchKernelGGL(k_copy, dim3(grid_for(5LL * g.n, kBlock)), dim3(kBlock), 0, st, s.delUp,

@@ -104,6 +104,24 @@ def cpu_baseline_sample(cfg, n_sample):
                       f"{dt:.1f} s"}
 
 
+def cpu_baseline_batch(clouds, plan):
+    """The C oracle on one core for the batch: its median-size cloud, the
+    first two iterations from the freestream (each cloud iterates on its own;
+    the batch's rate is the same work cloud after cloud)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    b = int(np.argsort([c.n for c in clouds])[len(clouds) // 2])
+    cl, p = clouds[b], plan[b]
+    prim0 = np.tile(pc.freestream_prim(p["mach"], p["aoa"]), (cl.n, 1))
+    o = orc.Oracle(cl, p["mach"], p["aoa"], pc.BATCH["cfl"], prim=prim0)
+    t0 = time.perf_counter()
+    o.run(1, 2, pc.BATCH["mode"])
+    dt = time.perf_counter() - t0
+    return {"value": round(2 * cl.n / dt / 1e6, 5), "unit": "Mpoint-iter/s", "cores": 1, "kind": "port",
+            "sample": f"iterations 1-2 of batch cloud {b} ({cl.n} points, the median size, Mach {p['mach']:.3f}, "
+                      f"AoA {p['aoa']:.2f} deg), C restatement of the reference on one host core, {dt:.1f} s"}
+
+
 def copy_probe(mib):
     """A plain device copy of known size (reads and writes mib MiB each
     launch, three launches) for scaling the HBM traffic counters."""
@@ -339,6 +357,7 @@ def bench_batch(args):
     flux_bytes = int(roofline.flux_bytes_per_point(status, cnt, nbhs)[f64["cls"] == 0].sum())
     achieved = flux_bytes / (f64["flux_ms"] * 1e-3) / 1e9
     last = f64["inst"][-1]
+    prof = read_profile_values("flux_interior_c5_batch_traffic.txt")
     mixed = None
     if "mixed" in legs:
         mx = legs["mixed"]
@@ -364,10 +383,11 @@ def bench_batch(args):
                                f"one HIP graph per iteration",
                    "points": n, "clouds": len(clouds), "mode": cfg["mode"], "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": roofline.HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / roofline.HBM_PEAK_GBS, 5),
+                     "traffic": prof.get("hbm_bytes_per_launch"),
                      "kernel": "k_flux_fused<0,true,double> (interior points on the fused path)",
                      "bytes_per_launch": flux_bytes, "launch_ms": round(f64["flux_ms"], 4)},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_batch(clouds, plan),
         "mixed": mixed,
         "tuning": f64["tuning"],
         "instance_residual_last": {"min": float(last.min()), "max": float(last.max()),

@@ -59,13 +59,39 @@ def pkg():
     return PKG
 
 
+def walked_empty_lists(cl, cnt):
+    """(list, point) pairs of empty direction lists that the flux walks:
+    interior points walk all six, wall points xpos..yneg and conn for zneg,
+    outer points xpos..zpos, supersonic points none."""
+    pc = PKG.pointcloud
+    out = []
+    for d in range(6):
+        for p in range(cl.n):
+            st = cl.status[p]
+            if st == pc.STATUS_WALL and d == 4:
+                continue
+            if st == pc.STATUS_WALL and d == 5:
+                n = cl.nbhs[p]
+            elif st == pc.STATUS_OUTER and d == 5:
+                continue
+            elif st not in (pc.STATUS_INTERIOR, pc.STATUS_WALL, pc.STATUS_OUTER):
+                continue
+            else:
+                n = cnt[d, p]
+            if n == 0:
+                out.append((d, p))
+    return out
+
+
 def degenerate_cloud(kind):
     """The explicit_blocks fixture cloud with one interior point's xpos
-    stencil made degenerate: kind "collinear" moves each of its xpos
+    stencil made degenerate.  Kind "collinear" moves each of its xpos
     neighbours onto the point's tan1 axis at its own distance along it (the
-    list's least-squares matrix becomes rank one), kind "empty" mirrors them
-    to the negative side (the list empties).  The point is chosen so that its
-    other five lists keep entries.  Returns (cloud, point index 0-based)."""
+    list's least-squares matrix becomes rank one); the point is one for which
+    no walked list of the cloud then empties.  Kind "empty" keeps the cloud
+    and hands over split lists in which that point's xpos count is zero (as
+    a host's own preprocessing could).  Returns (cloud, point index 0-based,
+    split lists or None)."""
     pc = PKG.pointcloud
     solver = PKG.solver
     cl, _ = load_fixture("explicit_blocks")
@@ -77,13 +103,16 @@ def degenerate_cloud(kind):
         ks = lists[0, 22 * p:22 * p + cnt[0, p]] - 1
         if np.any(cl.status[ks] != pc.STATUS_INTERIOR):
             continue
+        if kind == "empty":
+            c2 = cnt.copy()
+            c2[0, p] = 0
+            return cl, int(p), (c2, lists)
         q = p3.copy()
         dels = (p3[ks] - p3[p]) @ cl.tan1[p]
-        q[ks] = p3[p] + (dels if kind == "collinear" else -dels)[:, None] * cl.tan1[p][None, :]
+        q[ks] = p3[p] + dels[:, None] * cl.tan1[p][None, :]
         out = pc.PointCloud(q[:, 0].copy(), q[:, 1].copy(), q[:, 2].copy(), cl.tan1, cl.tan2, cl.nor,
                             cl.status, cl.nbhs, cl.conn, cl.min_dist)
         c2, _ = solver.split_stencils(out)
-        want_empty = [0] if kind == "empty" else []
-        if all((c2[d, p] == 0) == (d in want_empty) for d in range(6)):
-            return out, int(p)
+        if not walked_empty_lists(out, c2) and c2[0, p] == cnt[0, p]:
+            return out, int(p), None
     raise RuntimeError(f"no point of the fixture takes a {kind} xpos stencil")

@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
@@ -673,6 +674,34 @@ HostList host_list(const mfcfd_geometry* gm, int r, int d) {
     return {gm->split_conn[d] + 22LL * r, gm->split_nbhs[d][r]};
 }
 
+// The per-iteration residual history of mfcfd_run_noalloc: one double per
+// iteration, written by the state update's last workgroup, in a file-scope
+// device array so that the call allocates nothing.  Every device holds its
+// own copy for the life of the process, and host code reaches it only
+// through the address hipGetSymbolAddress returned with that device current
+// (the array's name in host code is not a device address).  Calls on one
+// device share the array: two contexts that run mfcfd_run_noalloc at the same
+// time on one device race.  Every entry a call reads is written by the same
+// call first, so nothing is cleared.
+static __device__ __align__(256) double scratch[MFCFD_NOALLOC_MAX_ITERS];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// once per device, with that device current (mfcfd_create)
+int scratch_address(int device) {
+    if (device >= kMaxDevices) return fail(MFCFD_EINVAL, "device %d beyond the %d supported", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return MFCFD_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(MFCFD_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch[device] = static_cast<double*>(p);
+    return MFCFD_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -688,6 +717,7 @@ int mfcfd_create(int device, mfcfd_ctx** out) {
     HIP_TRY(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) return fail(MFCFD_EINVAL, "device %d not present (%d devices)", device, count);
     HIP_TRY(hipSetDevice(device));
+    if (int rc = scratch_address(device)) return rc;
     mfcfd_ctx* c = new (std::nothrow) mfcfd_ctx();
     if (!c) return fail(MFCFD_ENOMEM, "context allocation failed");
     c->device = device;
@@ -1426,6 +1456,38 @@ int mfcfd_run(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, double* r
         residue_bookkeeping(c, t0 + it, h[it]);
         if (res_new_hist) res_new_hist[it] = c->res_new;
         if (residue_hist) residue_hist[it] = c->residue;
+    }
+    return MFCFD_OK;
+}
+
+int mfcfd_run_noalloc(mfcfd_ctx* c, int t0, int n_iters, double* res_new_hist, double* residue_hist) {
+    int rc = check_ctx(c, true);
+    if (rc) return rc;
+    if (n_iters < 0) return fail(MFCFD_EINVAL, "n_iters < 0");
+    if (n_iters > MFCFD_NOALLOC_MAX_ITERS)
+        return fail(MFCFD_EINVAL, "n_iters %d exceeds the no-allocation history of %d iterations", n_iters,
+                    MFCFD_NOALLOC_MAX_ITERS);
+    if (n_iters == 0) return MFCFD_OK;
+    if (c->time_kernel) return fail(MFCFD_ESTATE, "kernel timing creates events; disable it before mfcfd_run_noalloc");
+    if (c->params.mode != MFCFD_EXPLICIT && c->plan_form != c->params.mode)
+        return fail(MFCFD_ESTATE, "the sweep plan of this iteration form is not built (building it allocates): "
+                                  "call mfcfd_sweep_info, mfcfd_fpi_solver or mfcfd_run once first");
+    double* hist = g_scratch[c->device];
+    if (!hist) return fail(MFCFD_ESTATE, "no scratch address for device %d", c->device);
+    for (int it = 0; it < n_iters; ++it)
+        if ((rc = do_iteration(c, hist, it))) return rc;
+    if ((rc = fetch_record(c))) return rc;
+    // the history comes back through a fixed host buffer, a block at a time
+    double h[256];
+    for (int off = 0; off < n_iters; off += 256) {
+        const int m = std::min(256, n_iters - off);
+        HIP_TRY(hipMemcpyAsync(h, hist + off, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int j = 0; j < m; ++j) {
+            residue_bookkeeping(c, t0 + off + j, h[j]);
+            if (res_new_hist) res_new_hist[off + j] = c->res_new;
+            if (residue_hist) residue_hist[off + j] = c->residue;
+        }
     }
     return MFCFD_OK;
 }

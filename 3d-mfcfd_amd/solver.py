@@ -28,6 +28,7 @@ EXPLICIT, IMPLICIT, IMPLICIT_BACKUP = 0, 1, 2
 # state_update_implicit.f90)
 MODES = {"explicit": EXPLICIT, "implicit": IMPLICIT, "implicit_backup": IMPLICIT_BACKUP}
 FP64, MIXED = 0, 1
+NOALLOC_MAX_ITERS = 4096   # iterations of one mfcfd_run_noalloc call
 ARR = {"prim": 0, "U": 1, "q": 2, "qm": 3, "dq": 4, "delt": 5, "flux_res": 6, "delUp": 7,
        "delUn": 8}
 ARR_SHAPE = {"prim": (5,), "U": (5,), "q": (5,), "qm": (5, 2), "dq": (5, 3), "delt": (),
@@ -89,7 +90,7 @@ EXPORTS = ["mfcfd_create", "mfcfd_destroy", "mfcfd_last_error", "mfcfd_abi_versi
            "mfcfd_run", "mfcfd_get_residual", "mfcfd_synchronize", "mfcfd_sweep_info",
            "mfcfd_time_phases", "mfcfd_point_classes", "mfcfd_point_class_map",
            "mfcfd_kernel_timing", "mfcfd_kernel_time", "mfcfd_bind_batch", "mfcfd_batch_info",
-           "mfcfd_run_graph", "mfcfd_autotune", "mfcfd_set_tuning"]
+           "mfcfd_run_graph", "mfcfd_autotune", "mfcfd_set_tuning", "mfcfd_run_noalloc"]
 PHASES = ["conserved_q", "qderiv_timestep", "flux_interior", "flux_boundary", "sweeps",
           "state_update"]
 
@@ -119,6 +120,7 @@ def lib():
             getattr(L, nm).argtypes = [vp]
         L.mfcfd_fpi_solver.argtypes = [vp, C.c_int, _dp, _dp]
         L.mfcfd_run.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
+        L.mfcfd_run_noalloc.argtypes = [vp, C.c_int, C.c_int, _dp, _dp]
         L.mfcfd_get_residual.argtypes = [vp, C.POINTER(Residual)]
         L.mfcfd_sweep_info.argtypes = [vp, _ip, _ip, _ip]
         L.mfcfd_time_phases.argtypes = [vp, C.c_int, C.c_int, _dp]
@@ -288,6 +290,15 @@ class Solver:
         rn = np.zeros(max(n_iters, 1))
         rs = np.zeros(max(n_iters, 1))
         _check(lib().mfcfd_run(self._ctx, t0, n_iters, _d(rn), _d(rs)))
+        return rn[:n_iters], rs[:n_iters]
+
+    def run_noalloc(self, t0, n_iters):
+        """mfcfd_run without allocation (no counterpart in the reference):
+        at most NOALLOC_MAX_ITERS iterations, and the implicit form's sweep
+        plan must already be built (sweep_info, fpi_solver or run)."""
+        rn = np.zeros(max(n_iters, 1))
+        rs = np.zeros(max(n_iters, 1))
+        _check(lib().mfcfd_run_noalloc(self._ctx, t0, n_iters, _d(rn), _d(rs)))
         return rn[:n_iters], rs[:n_iters]
 
     def residual(self):

@@ -203,6 +203,19 @@ int mfcfd_fpi_solver(mfcfd_ctx *ctx, int t, double *res_new, double *residue);
  * round trip; per-iteration res_new and residue (either may be NULL) */
 int mfcfd_run(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, double *residue_hist);
 
+/* mfcfd_run that allocates nothing (no counterpart in the reference): for
+ * callers that cannot allocate once they are set up -- serving loops and
+ * harnesses that forbid hipMalloc after binding.  mfcfd_run grows a device history buffer with hipMalloc when n_iters
+ * exceeds every earlier call's; this variant keeps the per-iteration
+ * residuals in a fixed device array of MFCFD_NOALLOC_MAX_ITERS doubles (one
+ * per device, held until the process ends), returns MFCFD_EINVAL for longer
+ * runs, and MFCFD_ESTATE when the implicit form's sweep plan has not been
+ * built yet (mfcfd_sweep_info, mfcfd_fpi_solver or mfcfd_run builds it) or
+ * kernel timing is on.  Results are those of mfcfd_run.  Calls on one device
+ * share the array: concurrent calls from two contexts on one device race. */
+#define MFCFD_NOALLOC_MAX_ITERS 4096
+int mfcfd_run_noalloc(mfcfd_ctx *ctx, int t0, int n_iters, double *res_new_hist, double *residue_hist);
+
 /* mfcfd_run with the iteration captured once as a HIP graph and replayed
  * n_iters times; res_new_hist as in mfcfd_run (the whole bound set), and
  * inst_res_hist[it*n_instances + b] the residual of cloud b alone

@@ -341,7 +341,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 T prim_old[5], Gold[5];
 #pragma unroll
                 for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k + r]);
-                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
+                flux_eval<T, true>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
                 double pk[5];
                 if (packed & kTakeWait) {
                     wait_read(a.pub + 5 * k, pk, a.stalls, a.zero);
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 T pt[5], Gnew[5];
 #pragma unroll
                 for (int r = 0; r < 5; ++r) pt[r] = T(pk[r]);
-                flux_eval<T>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
+                flux_eval<T, true>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
 #pragma unroll
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }

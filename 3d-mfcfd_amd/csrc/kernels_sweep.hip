@@ -164,24 +164,35 @@ __device__ __forceinline__ double dpp_move(double v) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// Sum over the 64 lanes of a wave (all active), in a fixed order, returned
-// to every lane.  Prefix sums within each row of 16 lanes (row_shr 1, 2, 4,
-// 8: lane 15 of a row holds the row's sum), row 0's sum into row 1 and row
-// 2's into row 3 (row_bcast:15), then lane 31's into rows 2 and 3
-// (row_bcast:31): lane 63 ends with the total, read back with readlane.
-// Each step is one VALU move per half, where a shuffle is a round trip
-// through the LDS crossbar.
-__device__ __forceinline__ double wave_sum(double v) {
-    v = v + dpp_move<0x111, 0xf>(v);
-    v = v + dpp_move<0x112, 0xf>(v);
-    v = v + dpp_move<0x114, 0xf>(v);
-    v = v + dpp_move<0x118, 0xf>(v);
-    v = v + dpp_move<0x142, 0xa>(v);
-    v = v + dpp_move<0x143, 0xc>(v);
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, 63);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+// Sums over the 64 lanes of a wave (all active), in a fixed order, returned
+// to every lane, for five values at once.  Prefix sums within each row of 16
+// lanes (row_shr 1, 2, 4, 8: lane 15 of a row holds the row's sum), row 0's
+// sum into row 1 and row 2's into row 3 (row_bcast:15), then lane 31's into
+// rows 2 and 3 (row_bcast:31): lane 63 ends with the total, read back with
+// readlane.  Each step is one VALU move per half, where a shuffle is a round
+// trip through the LDS crossbar.  The five values take each step together,
+// so five independent chains are in flight instead of one: the reduction
+// sits on the sweep's dependency chain.
+__device__ __forceinline__ void wave_sum5(double v[5]) {
+#pragma unroll
+    for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x111, 0xf>(v[r]);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x112, 0xf>(v[r]);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x114, 0xf>(v[r]);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x118, 0xf>(v[r]);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x142, 0xa>(v[r]);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x143, 0xc>(v[r]);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const long long b = __double_as_longlong(v[r]);
+        const int lo = __builtin_amdgcn_readlane((int)b, 63);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+        v[r] = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    }
 }
 
 // the list a sweep walks for point i and direction d (SweepPlan's
@@ -335,9 +346,10 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 const long long k = (long long)(packed >> kTakeShift);
                 const int d = (int)(packed & 7u);
                 const double wgt = a.wgt[e];
-                // the neighbour's state of this sweep first: nothing is held
-                // across the wait (a point's dependencies have mostly
-                // finished before its wave reaches them), then both fluxes
+                // the flux of the neighbour's state before the sweep first:
+                // it does not depend on the sweep, so its loads and
+                // arithmetic overlap the wait for the state of this sweep,
+                // whose flux follows
                 T prim_old[5], Gold[5];
 #pragma unroll
                 for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k + r]);
@@ -356,8 +368,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
 #pragma unroll
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }
-#pragma unroll
-            for (int r = 0; r < 5; ++r) acc[r] = wave_sum(acc[r]);
+            wave_sum5(acc);
             if (lane == 0) {
                 // wall terms are doubled (implicit_LdelU_forward_sweep_wall.f90:405)
 #pragma unroll

@@ -39,9 +39,12 @@
 //                  published primitive state -- polled until it is no
 //                  longer pending for a wait entry, read from pold for an
 //                  old one -- and Gnew from it.  The wave's sum is a
-//                  fixed-order DPP reduction.  Lane 0 stores the point's delU
-//                  and its published state prim(U_i + delU_i), computed once
-//                  here instead of once per entry that reads it.
+//                  fixed-order DPP reduction.  Lane 0 publishes the point's
+//                  state prim(U_i + delU_i), computed once here instead of
+//                  once per entry that reads it, then stores its delU.  The
+//                  mixed form publishes that state as five FP32 values in
+//                  three 64-bit words: its consumers evaluate their fluxes
+//                  in FP32, so three words per poll instead of five.
 //
 // Memory model.  Inside k_sweep_async the memory other waves read while it
 // runs is pub (and dsrc, the published delU of the copy sources), each 64-bit
@@ -69,8 +72,11 @@
 // exceeds the spin limit is counted in the residual record and the host
 // reports an error; the kernel still runs to completion.
 //
-// T is the arithmetic of the split fluxes (float in the mixed form); the
-// weights, the sums, D_inv and the new delU stay FP64.
+// T is the arithmetic of the split fluxes and of the published state (float
+// in the mixed form); the weights, the sums, D_inv and the new delU stay FP64.
+// The FP64 flux evaluations take the sweep form of qlskum_math.h (flux_eval
+// with SWEEP): every Gold and Gnew of both sweeps is evaluated the same way,
+// so an unchanged neighbour contributes exactly zero.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -111,14 +117,15 @@ __device__ __forceinline__ void store_value(double* p, double v) {
 // of zero after kSpinAtomic polls.  Gives up after the spin limit, or at once
 // when another wait of this sweep already gave up, so a broken schedule ends
 // the kernel instead of stalling it point by point.
-__device__ __forceinline__ void wait_read(double* v5, double out[5], int* stalls, unsigned long long zero) {
-    unsigned long long* w = reinterpret_cast<unsigned long long*>(v5);
+template <int NW>
+__device__ __forceinline__ void wait_words(double* v, unsigned long long out[NW], int* stalls,
+                                           unsigned long long zero) {
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(v);
     bool ok = true;
 #pragma unroll
-    for (int r = 0; r < 5; ++r) {
-        const unsigned long long b = __hip_atomic_load(w + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        out[r] = __longlong_as_double((long long)b);
-        ok = ok && b != kPending;
+    for (int r = 0; r < NW; ++r) {
+        out[r] = __hip_atomic_load(w + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = ok && out[r] != kPending;
     }
     if (ok) return;
     int spins = 0;
@@ -134,16 +141,40 @@ __device__ __forceinline__ void wait_read(double* v5, double out[5], int* stalls
         __builtin_amdgcn_s_sleep(1);
         ok = true;
 #pragma unroll
-        for (int r = 0; r < 5; ++r) {
-            const unsigned long long b =
-                spins < kSpinAtomic
-                    ? __hip_atomic_load(w + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                    : __hip_atomic_fetch_add(w + r, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            out[r] = __longlong_as_double((long long)b);
-            ok = ok && b != kPending;
+        for (int r = 0; r < NW; ++r) {
+            out[r] = spins < kSpinAtomic
+                         ? __hip_atomic_load(w + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : __hip_atomic_fetch_add(w + r, zero, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = ok && out[r] != kPending;
         }
         if (ok) return;
     }
+}
+
+__device__ __forceinline__ void wait_read(double* v5, double out[5], int* stalls, unsigned long long zero) {
+    unsigned long long b[5];
+    wait_words<5>(v5, b, stalls, zero);
+#pragma unroll
+    for (int r = 0; r < 5; ++r) out[r] = __longlong_as_double((long long)b[r]);
+}
+
+// The mixed form publishes prim(U + delU) as five FP32 values in three 64-bit
+// words (its consumers evaluate their fluxes in FP32 anyway): lo/hi halves
+// p0/p1, p2/p3, p4/0.  A word with a zero high half never matches kPending,
+// and the other two would need the FP32 NaN 0x7FF4DEAD, which arithmetic does
+// not produce.
+__device__ __forceinline__ unsigned long long pack2(float lo, float hi) {
+    return (unsigned long long)__float_as_uint(lo) | ((unsigned long long)__float_as_uint(hi) << 32);
+}
+
+__device__ __forceinline__ void wait_read_packed(double* v, float out[5], int* stalls, unsigned long long zero) {
+    unsigned long long b[3];
+    wait_words<3>(v, b, stalls, zero);
+    out[0] = __uint_as_float((unsigned)b[0]);
+    out[1] = __uint_as_float((unsigned)(b[0] >> 32));
+    out[2] = __uint_as_float((unsigned)b[1]);
+    out[3] = __uint_as_float((unsigned)(b[1] >> 32));
+    out[4] = __uint_as_float((unsigned)b[2]);
 }
 
 // a 64-bit value every lane of the wave holds alike, in scalar registers
@@ -354,26 +385,32 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
 #pragma unroll
                 for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k + r]);
                 flux_eval<T, true>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, prim_old, T(fp.pi), Gold);
-                double pk[5];
+                T pt[5], Gnew[5];
                 if (packed & kTakeWait) {
-                    wait_read(a.pub + 5 * k, pk, a.stalls, a.zero);
+                    if constexpr (sizeof(T) == 4) {
+                        wait_read_packed(a.pub + 5 * k, pt, a.stalls, a.zero);
+                    } else {
+                        double pk[5];
+                        wait_read(a.pub + 5 * k, pk, a.stalls, a.zero);
+#pragma unroll
+                        for (int r = 0; r < 5; ++r) pt[r] = T(pk[r]);
+                    }
                 } else {
 #pragma unroll
-                    for (int r = 0; r < 5; ++r) pk[r] = a.pold[5 * k + r];
+                    for (int r = 0; r < 5; ++r) pt[r] = T(a.pold[5 * k + r]);
                 }
-                T pt[5], Gnew[5];
-#pragma unroll
-                for (int r = 0; r < 5; ++r) pt[r] = T(pk[r]);
                 flux_eval<T, true>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
 #pragma unroll
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }
             wave_sum5(acc);
             if (lane == 0) {
-                // wall terms are doubled (implicit_LdelU_forward_sweep_wall.f90:405)
+                // wall terms are doubled (implicit_LdelU_forward_sweep_wall.f90:405);
+                // the product with 1 or 2 is exact
+                const double twice = cls == 1 ? 2.0 : 1.0;
 #pragma unroll
                 for (int r = 0; r < 5; ++r) {
-                    const double term = cls == 1 ? 2.0 * acc[r] : acc[r];
+                    const double term = twice * acc[r];
                     res[r] = FORWARD ? -(D_inv * (own[r] + term)) : own[r] - D_inv * term;
                 }
             }
@@ -381,18 +418,31 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         if (lane == 0) {
             double U[5], pp[5];
 #pragma unroll
-            for (int r = 0; r < 5; ++r) {
-                dU[5 * (long long)i + r] = res[r];
-                U[r] = res[r] + Ui[r];
-            }
-            conserved_to_prim(U, f, pp);
+            for (int r = 0; r < 5; ++r) U[r] = res[r] + Ui[r];
+            if constexpr (sizeof(T) == 4) {
+                // the mixed form: converted and published in FP32, packed
+                const Frame<float> f32 = frame_as<float>(f);
+                float U32[5], p32[5];
 #pragma unroll
-            for (int r = 0; r < 5; ++r) store_value(a.pub + 5 * (long long)i + r, pp[r]);
+                for (int r = 0; r < 5; ++r) U32[r] = float(U[r]);
+                conserved_to_prim(U32, f32, p32);
+                unsigned long long* w = reinterpret_cast<unsigned long long*>(a.pub + 5 * (long long)i);
+                __hip_atomic_exchange(w, pack2(p32[0], p32[1]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_exchange(w + 1, pack2(p32[2], p32[3]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_exchange(w + 2, pack2(p32[4], 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                conserved_to_prim(U, f, pp);
+#pragma unroll
+                for (int r = 0; r < 5; ++r) store_value(a.pub + 5 * (long long)i + r, pp[r]);
+            }
             // a copy source that a supersonic point reads as this sweep's value
             if (slot >= 0) {
 #pragma unroll
                 for (int r = 0; r < 5; ++r) store_value(a.dsrc + 5 * (long long)slot + r, res[r]);
             }
+            // delU is read by later kernels only: stored after the publication
+#pragma unroll
+            for (int r = 0; r < 5; ++r) dU[5 * (long long)i + r] = res[r];
         }
     }
     // the last workgroup out resets the counters for the next sweep

@@ -3,7 +3,9 @@
 // Every routine keeps the reference's expression order (Fortran a+b+c is
 // (a+b)+c, a*b*c is (a*b)*c) and the FP64 kernels are compiled with
 // -ffp-contract=off, so the only differences from the reference CPU build
-// come from the device exp/log/erf (ocml vs glibc, within an ulp or two).
+// come from the device exp/log/erf (ocml vs glibc, within an ulp or two) --
+// and, in the sync-free sweeps, from the sweep form of the split fluxes
+// (SWEEP below: two roots from one reciprocal square root, within 2 ulp).
 // Routines are templated on the arithmetic type T so the mixed-precision
 // kernels reuse them with float, where division, square root and erf take
 // the faster forms defined below.
@@ -48,12 +50,47 @@ template <typename T> __device__ __forceinline__ T derf_e(T x, T ex);
 // from ~150 to ~223 VGPRs (2 instead of 3 resident waves per SIMD)
 __device__ __attribute__((noinline)) inline double erf_out_of_line(double x) { return erf(x); }
 template <> __device__ __forceinline__ double derf_e<double>(double x, double) { return erf_out_of_line(x); }
-// INLINE_ERF: the library erf inlined, for a kernel with registers to spare
-// whose flux evaluation sits on a dependency chain (the sweeps: 3% shorter)
-template <typename T, bool INLINE_ERF>
+// SWEEP selects the sweep form of the split fluxes, for kernels whose flux
+// evaluations sit on a dependency chain.  In FP64 it
+//   * inlines the library erf (registers are spare there: 3% shorter), and
+//   * takes sqrt(beta) and 0.5/sqrt(pi*beta) from one reciprocal square root
+//     of beta (beta_roots), so that the coefficient B is one multiply after
+//     exp instead of an IEEE division (within 2 ulp of the IEEE forms).
+// FP32 is the same in both forms.
+template <typename T, bool SWEEP>
 __device__ __forceinline__ T derf_sel(T x, T ex) {
-    if constexpr (INLINE_ERF && sizeof(T) == 8) return erf(x);
+    if constexpr (SWEEP && sizeof(T) == 8) return erf(x);
     else return derf_e(x, ex);
+}
+
+// 1/sqrt(x) to double precision: the hardware estimate (v_rsq_f64) refined
+// by two Newton-Raphson steps y <- y*(1.5 - 0.5*x*y*y); x is a positive
+// normal number
+__device__ __forceinline__ double rsqrt_refined(double x) {
+    const double hx = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-hx, y * y, 1.5);
+    y = y * fma(-hx, y * y, 1.5);
+    return y;
+}
+
+// sb = sqrt(beta) and hb = 0.5/sqrt(pi*beta), so that B = exp(-S*S)*hb
+template <typename T, bool SWEEP>
+__device__ __forceinline__ void beta_roots(T beta, T pi, T& sb, T& hb) {
+    if constexpr (SWEEP && sizeof(T) == 8) {
+        const T y = rsqrt_refined(beta);
+        sb = beta * y;
+        hb = (T(0.5) * rsqrt_refined(pi)) * y;
+    } else {
+        sb = dsqrt_fast(beta);
+        hb = T(0);   // unused: the reference form divides by sqrt(pi*beta)
+    }
+}
+
+template <typename T, bool SWEEP>
+__device__ __forceinline__ T b_coef(T ex, T hb, T pi, T beta) {
+    if constexpr (SWEEP && sizeof(T) == 8) return ex * hb;
+    else return ddiv(T(0.5) * ex, dsqrt_fast(pi * beta));
 }
 template <> __device__ __forceinline__ float derf_e<float>(float x, float ex) {
     const float ax = x < 0.0f ? -x : x;
@@ -113,7 +150,7 @@ struct Frame {
 // Half-range KFVS flux of split_fluxes.f90:11-308 along axis 0/1/2
 // (tan1/tan2/nor), half-space sign +1/-1.  "x - y" of the negative variants
 // is written "x + (-y)", which rounds identically.
-template <typename T, bool INLINE_ERF = false>
+template <typename T, bool SWEEP = false>
 __device__ __forceinline__ void flux_half(int axis, int sign, const Frame<T>& f, const T prim[5],
                                           T pi, T G[5]) {
     const T rho = prim[0], u1 = prim[1], u2 = prim[2], u3 = prim[3], pr = prim[4];
@@ -122,10 +159,12 @@ __device__ __forceinline__ void flux_half(int axis, int sign, const Frame<T>& f,
     const T un = f.n[0] * u1 + f.n[1] * u2 + f.n[2] * u3;
     const T beta = ddiv(T(0.5) * rho, pr);
     const T ua = axis == 0 ? ut1 : (axis == 1 ? ut2 : un);
-    const T S = ua * dsqrt_fast(beta);
+    T sb, hb;
+    beta_roots<T, SWEEP>(beta, pi, sb, hb);
+    const T S = ua * sb;
     const T ex = dexp(-S * S);
-    const T B = ddiv(T(0.5) * ex, dsqrt_fast(pi * beta));
-    const T e = derf_sel<T, INLINE_ERF>(S, ex);
+    const T B = b_coef<T, SWEEP>(ex, hb, pi, beta);
+    const T e = derf_sel<T, SWEEP>(S, ex);
     const T A = T(0.5) * (sign > 0 ? T(1) + e : T(1) - e);
     const T Bs = sign > 0 ? B : -B;
     const T u_sqr = ut1 * ut1 + ut2 * ut2 + un * un;
@@ -142,7 +181,7 @@ __device__ __forceinline__ void flux_half(int axis, int sign, const Frame<T>& f,
 
 // Quarter-range fluxes of octant_fluxes.f90:9-442: axis 0/1, sign_a along
 // that axis, sign_n = -1 wall family (A3neg, -B3), +1 outer family (A3pos).
-template <typename T, bool INLINE_ERF = false>
+template <typename T, bool SWEEP = false>
 __device__ __forceinline__ void flux_octant(int axis, int sign_a, int sign_n, const Frame<T>& f,
                                             const T prim[5], T pi, T G[5]) {
     const T rho = prim[0], u1 = prim[1], u2 = prim[2], u3 = prim[3], pr = prim[4];
@@ -151,14 +190,16 @@ __device__ __forceinline__ void flux_octant(int axis, int sign_a, int sign_n, co
     const T un = f.n[0] * u1 + f.n[1] * u2 + f.n[2] * u3;
     const T beta = ddiv(T(0.5) * rho, pr);
     const T ua = axis == 0 ? ut1 : ut2;
-    const T Sa = ua * dsqrt_fast(beta);
-    const T S3 = un * dsqrt_fast(beta);
+    T sb, hb;
+    beta_roots<T, SWEEP>(beta, pi, sb, hb);
+    const T Sa = ua * sb;
+    const T S3 = un * sb;
     const T exa = dexp(-Sa * Sa), ex3 = dexp(-S3 * S3);
-    const T Ba = ddiv(T(0.5) * exa, dsqrt_fast(pi * beta));
-    const T B3 = ddiv(T(0.5) * ex3, dsqrt_fast(pi * beta));
-    const T ea = derf_sel<T, INLINE_ERF>(Sa, exa);
+    const T Ba = b_coef<T, SWEEP>(exa, hb, pi, beta);
+    const T B3 = b_coef<T, SWEEP>(ex3, hb, pi, beta);
+    const T ea = derf_sel<T, SWEEP>(Sa, exa);
     const T Aa = T(0.5) * (sign_a > 0 ? T(1) + ea : T(1) - ea);
-    const T e3 = derf_sel<T, INLINE_ERF>(S3, ex3);
+    const T e3 = derf_sel<T, SWEEP>(S3, ex3);
     const T A3 = T(0.5) * (sign_n > 0 ? T(1) + e3 : T(1) - e3);
     const T u_sqr = ut1 * ut1 + ut2 * ut2 + un * un;
     const T temp1 = (ua * Aa + (sign_a > 0 ? Ba : -Ba));
@@ -179,13 +220,13 @@ __device__ __forceinline__ void flux_octant(int axis, int sign_a, int sign_n, co
 }
 
 // flux family selector: 0 half-range, 1 wall octant (Gw), 2 outer octant (Go)
-template <typename T, bool INLINE_ERF = false>
+template <typename T, bool SWEEP = false>
 __device__ __forceinline__ void flux_eval(int family, int axis, int sign, const Frame<T>& f,
                                           const T prim[5], T pi, T G[5]) {
     if (family == 0)
-        flux_half<T, INLINE_ERF>(axis, sign, f, prim, pi, G);
+        flux_half<T, SWEEP>(axis, sign, f, prim, pi, G);
     else
-        flux_octant<T, INLINE_ERF>(axis, sign, family == 2 ? 1 : -1, f, prim, pi, G);
+        flux_octant<T, SWEEP>(axis, sign, family == 2 ? 1 : -1, f, prim, pi, G);
 }
 
 // limiters.f90:18-40 for one component, from del_neg = qtilde - q

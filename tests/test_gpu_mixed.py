@@ -72,7 +72,10 @@ def test_mixed_follows_fp64_on_stretched_cloud(mode, cfl, iters):
     mach, aoa = 0.84, 3.06
     r64, p64 = run(cl, mach, aoa, cfl, iters, mode)
     r32, p32 = run(cl, mach, aoa, cfl, iters, mode, "mixed")
+    # from iteration 2: the first residual from the freestream is zero or
+    # round-off in both forms (round-off of FP32 fluxes in the mixed one)
     nz = r64 != 0
+    nz[0] = False
     herr = float(np.max(np.abs(r32[nz] - r64[nz]) / r64[nz]))
     e = cp_err(p32, p64, cl, mach)
     note(f"      residual history max rel diff {herr:.2e}, wall Cp rel err {e:.2e}")

@@ -131,6 +131,25 @@ def test_freestream_gives_zero_flux_residual_exactly():
     assert np.all(o.flux_res == 0.0)
 
 
+def test_first_implicit_iteration_moves_density_by_round_off_only():
+    # From the freestream the flux residual is zero, yet the sweeps see
+    # round-off: prim(U) does not reproduce prim bit for bit (the frame
+    # rotation and the pressure), so some Gnew - Gold are ulps.  The density
+    # corrections that result are nonzero but below half an ulp of the
+    # freestream density, which is why the reference's first residual
+    # (delta rho after rounding, state_update.f90:39-44) is exactly zero.
+    cl, d = load_fixture("implicit_morton")
+    o = orc.Oracle(cl, float(d["mach"]), float(d["aoa"]), float(d["cfl"]), prim=d["prim0"])
+    rn, _ = o.run(1, 1, True)
+    assert rn[0] == 0.0 and float(d["hist"][0, 0]) == 0.0
+    assert np.all(o.flux_res == 0.0)
+    swept = cl.status != 2
+    drho = o.delUn.reshape(-1, 5)[swept, 0]
+    rho = float(d["prim0"][0, 0])
+    assert np.count_nonzero(drho) > 0
+    assert np.max(np.abs(drho)) < 0.5 * np.spacing(rho)
+
+
 def test_ubar_at_freestream_is_freestream():
     # conserved_vector_Ubar.f90: A3+ + A3- = 1 and the B3 terms cancel
     cl, o = _tiny_oracle(mach=0.84, aoa=3.06)

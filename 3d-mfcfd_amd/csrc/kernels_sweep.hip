@@ -39,9 +39,10 @@
 //                  published primitive state -- polled until it is no
 //                  longer pending for a wait entry, read from pold for an
 //                  old one -- and Gnew from it.  The wave's sum is a
-//                  fixed-order DPP reduction.  Lane 0 publishes the point's
-//                  state prim(U_i + delU_i), computed once here instead of
-//                  once per entry that reads it, then stores its delU.  The
+//                  fixed-order DPP reduction that leaves the sums in lane
+//                  kTailLane; that lane publishes the point's state
+//                  prim(U_i + delU_i), computed once here instead of once
+//                  per entry that reads it, then stores its delU.  The
 //                  mixed form publishes that state as five FP32 values in
 //                  three 64-bit words: its consumers evaluate their fluxes
 //                  in FP32, so three words per poll instead of five.
@@ -195,15 +196,20 @@ __device__ __forceinline__ double dpp_move(double v) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// Sums over the 64 lanes of a wave (all active), in a fixed order, returned
-// to every lane, for five values at once.  Prefix sums within each row of 16
+// The lane that finishes a position.  The reduction below leaves the wave's
+// sums in lane 63, so the point's own inputs are loaded into that lane too,
+// and nothing crosses lanes between the reduction and the publication.
+constexpr int kTailLane = 63;
+
+// Sums over the 64 lanes of a wave (all active), in a fixed order, for five
+// values at once, left in lane kTailLane.  Prefix sums within each row of 16
 // lanes (row_shr 1, 2, 4, 8: lane 15 of a row holds the row's sum), row 0's
 // sum into row 1 and row 2's into row 3 (row_bcast:15), then lane 31's into
-// rows 2 and 3 (row_bcast:31): lane 63 ends with the total, read back with
-// readlane.  Each step is one VALU move per half, where a shuffle is a round
-// trip through the LDS crossbar.  The five values take each step together,
-// so five independent chains are in flight instead of one: the reduction
-// sits on the sweep's dependency chain.
+// rows 2 and 3 (row_bcast:31): lane 63 ends with the total.  Each step is one
+// VALU move per half, where a shuffle is a round trip through the LDS
+// crossbar.  The five values take each step together, so five independent
+// chains are in flight instead of one: the reduction sits on the sweep's
+// dependency chain.
 __device__ __forceinline__ void wave_sum5(double v[5]) {
 #pragma unroll
     for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x111, 0xf>(v[r]);
@@ -217,13 +223,41 @@ __device__ __forceinline__ void wave_sum5(double v[5]) {
     for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x142, 0xa>(v[r]);
 #pragma unroll
     for (int r = 0; r < 5; ++r) v[r] = v[r] + dpp_move<0x143, 0xc>(v[r]);
+}
+
+// The end of a position, on lane kTailLane: U_i + delU and its primitive
+// state, published (FP64, or FP32 packed in three words in the mixed form),
+// delU published too when the point is a copy source read in this sweep,
+// then delU stored for the later kernels.  Called at the end of both kinds
+// of position, so that no values merge between them before it.
+template <typename T>
+__device__ __forceinline__ void finish_position(const SweepAsync& a, double* dU, int i, const Frame<double>& f,
+                                                const double res[5], const double Ui[5], int slot) {
+    double U[5];
 #pragma unroll
-    for (int r = 0; r < 5; ++r) {
-        const long long b = __double_as_longlong(v[r]);
-        const int lo = __builtin_amdgcn_readlane((int)b, 63);
-        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
-        v[r] = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    for (int r = 0; r < 5; ++r) U[r] = res[r] + Ui[r];
+    if constexpr (sizeof(T) == 4) {
+        const Frame<float> f32 = frame_as<float>(f);
+        float U32[5], p32[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) U32[r] = float(U[r]);
+        conserved_to_prim(U32, f32, p32);
+        unsigned long long* w = reinterpret_cast<unsigned long long*>(a.pub + 5 * (long long)i);
+        __hip_atomic_exchange(w, pack2(p32[0], p32[1]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_exchange(w + 1, pack2(p32[2], p32[3]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_exchange(w + 2, pack2(p32[4], 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        double pp[5];
+        conserved_to_prim(U, f, pp);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) store_value(a.pub + 5 * (long long)i + r, pp[r]);
     }
+    if (slot >= 0) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) store_value(a.dsrc + 5 * (long long)slot + r, res[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) dU[5 * (long long)i + r] = res[r];
 }
 
 // the list a sweep walks for point i and direction d (SweepPlan's
@@ -337,12 +371,11 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         const int i = __builtin_amdgcn_readfirstlane(a.seq[t]);
         const long long tb = uniform64(a.take_ptr[t]), te = uniform64(a.take_ptr[t + 1]);
         const Frame<double> f = load_frame(g.frame, i);
-        double res[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         // the point's conserved state for the final update, loaded now so
         // that its latency stays off the path after the reduction
         double Ui[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         int slot = -1;   // the point's slot as a copy source read in this sweep
-        if (lane == 0) {
+        if (lane == kTailLane) {
 #pragma unroll
             for (int r = 0; r < 5; ++r) Ui[r] = s.U[5 * (long long)i + r];
             if (a.src_slot) slot = a.src_slot[i];
@@ -351,21 +384,23 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
         if (te > tb && (first & 7u) == kTakeCopy) {
             // supersonic point: the nearest neighbour's delU
             // (implicit_forward_and_reverse_sweeps.f90:38-52, 69-83)
-            if (lane == 0) {
+            if (lane == kTailLane) {
                 const long long q = (long long)(first >> kTakeShift);
+                double res[5];
                 if (first & kTakeWait) {
                     wait_read(a.dsrc + 5 * (long long)a.src_slot[q], res, a.stalls, a.zero);
                 } else {
 #pragma unroll
                     for (int r = 0; r < 5; ++r) res[r] = a.dold[5 * q + r];
                 }
+                finish_position<T>(a, dU, i, f, res, Ui, slot);
             }
         } else {
             const int cls = __builtin_amdgcn_readfirstlane(g.status[i]);
             const Frame<T> ft = frame_as<T>(f);
             // the point's own inputs of the final update, read before the wait
             double D_inv = 0.0, own[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            if (lane == 0) {
+            if (lane == kTailLane) {
                 D_inv = s.dinv[i];
 #pragma unroll
                 for (int r = 0; r < 5; ++r)
@@ -404,45 +439,18 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }
             wave_sum5(acc);
-            if (lane == 0) {
+            if (lane == kTailLane) {
                 // wall terms are doubled (implicit_LdelU_forward_sweep_wall.f90:405);
                 // the product with 1 or 2 is exact
                 const double twice = cls == 1 ? 2.0 : 1.0;
+                double res[5];
 #pragma unroll
                 for (int r = 0; r < 5; ++r) {
                     const double term = twice * acc[r];
                     res[r] = FORWARD ? -(D_inv * (own[r] + term)) : own[r] - D_inv * term;
                 }
+                finish_position<T>(a, dU, i, f, res, Ui, slot);
             }
-        }
-        if (lane == 0) {
-            double U[5], pp[5];
-#pragma unroll
-            for (int r = 0; r < 5; ++r) U[r] = res[r] + Ui[r];
-            if constexpr (sizeof(T) == 4) {
-                // the mixed form: converted and published in FP32, packed
-                const Frame<float> f32 = frame_as<float>(f);
-                float U32[5], p32[5];
-#pragma unroll
-                for (int r = 0; r < 5; ++r) U32[r] = float(U[r]);
-                conserved_to_prim(U32, f32, p32);
-                unsigned long long* w = reinterpret_cast<unsigned long long*>(a.pub + 5 * (long long)i);
-                __hip_atomic_exchange(w, pack2(p32[0], p32[1]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_exchange(w + 1, pack2(p32[2], p32[3]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_exchange(w + 2, pack2(p32[4], 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                conserved_to_prim(U, f, pp);
-#pragma unroll
-                for (int r = 0; r < 5; ++r) store_value(a.pub + 5 * (long long)i + r, pp[r]);
-            }
-            // a copy source that a supersonic point reads as this sweep's value
-            if (slot >= 0) {
-#pragma unroll
-                for (int r = 0; r < 5; ++r) store_value(a.dsrc + 5 * (long long)slot + r, res[r]);
-            }
-            // delU is read by later kernels only: stored after the publication
-#pragma unroll
-            for (int r = 0; r < 5; ++r) dU[5 * (long long)i + r] = res[r];
         }
     }
     // the last workgroup out resets the counters for the next sweep

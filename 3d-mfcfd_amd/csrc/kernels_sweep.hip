@@ -358,6 +358,8 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
     const unsigned wave = threadIdx.x >> 6;
     double* dU = FORWARD ? s.delUp : s.delUn;
     for (unsigned round = 0;; ++round) {
+        // a position's work up to its waits is off the dependency chain
+        __builtin_amdgcn_s_setprio(0);
         if (threadIdx.x == 0)
             sh_base[round & 1] = __hip_atomic_fetch_add(a.ctr, (unsigned)kSweepWaves, __ATOMIC_RELAXED,
                                                          __HIP_MEMORY_SCOPE_AGENT);
@@ -434,6 +436,10 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
 #pragma unroll
                     for (int r = 0; r < 5; ++r) pt[r] = T(a.pold[5 * k + r]);
                 }
+                // from here to the publication the wave is on the dependency
+                // chain: its SIMD issues for it before a wave still working
+                // ahead of its waits
+                __builtin_amdgcn_s_setprio(3);
                 flux_eval<T, true>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
 #pragma unroll
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;

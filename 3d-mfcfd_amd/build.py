@@ -3,7 +3,11 @@
 Every HIP translation unit is compiled for gfx950 only, with packed FP32
 instructions disabled; the FP64 parity kernels additionally use
 -ffp-contract=off so no multiply-add is fused that the reference does not
-fuse.  Objects go to 3d-mfcfd_amd/_build/, the library to
+fuse.  The sync-free sweep's unit is the exception: its sweep form of the
+split fluxes already rounds differently from the reference, and its
+flux evaluations sit on the sweep's dependency chain, so it fuses the
+multiply-adds within an expression (-ffp-contract=on), the same way at every
+call site.  Objects go to 3d-mfcfd_amd/_build/, the library to
 3d-mfcfd_amd/libmfcfd_hip.so.
 """
 
@@ -29,7 +33,7 @@ UNITS = {
     "capi.hip": ["-ffp-contract=off"],
     "kernels_fp64.hip": ["-ffp-contract=off"],
     "kernels_flux_fused.hip": ["-ffp-contract=off"],
-    "kernels_sweep.hip": ["-ffp-contract=off"],
+    "kernels_sweep.hip": ["-ffp-contract=on"],
     "preprocess.cpp": ["-ffp-contract=off", "-x", "hip"],
 }
 

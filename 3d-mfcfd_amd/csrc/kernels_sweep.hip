@@ -76,8 +76,9 @@
 // T is the arithmetic of the split fluxes and of the published state (float
 // in the mixed form); the weights, the sums, D_inv and the new delU stay FP64.
 // The FP64 flux evaluations take the sweep form of qlskum_math.h (flux_eval
-// with SWEEP): every Gold and Gnew of both sweeps is evaluated the same way,
-// so an unchanged neighbour contributes exactly zero.
+// with SWEEP), and this unit is compiled with -ffp-contract=on (a*b + c
+// fused within an expression): every Gold and Gnew of both sweeps is
+// evaluated the same way, so an unchanged neighbour contributes exactly zero.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"

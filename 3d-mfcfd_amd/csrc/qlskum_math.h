@@ -5,7 +5,9 @@
 // -ffp-contract=off, so the only differences from the reference CPU build
 // come from the device exp/log/erf (ocml vs glibc, within an ulp or two) --
 // and, in the sync-free sweeps, from the sweep form of the split fluxes
-// (SWEEP below: two roots from one reciprocal square root, within 2 ulp).
+// (SWEEP below: two roots from one reciprocal square root, within 2 ulp)
+// and from kernels_sweep.hip's -ffp-contract=on (a*b + c fused within an
+// expression, identically at every call site).
 // Routines are templated on the arithmetic type T so the mixed-precision
 // kernels reuse them with float, where division, square root and erf take
 // the faster forms defined below.

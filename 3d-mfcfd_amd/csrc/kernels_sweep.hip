@@ -112,8 +112,8 @@ __device__ __forceinline__ void store_value(double* p, double v) {
                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// read five values written in this sweep, waiting until none is pending:
-// agent-scope atomic loads of all five at every poll (polling one value and
+// read NW 64-bit words written in this sweep, waiting until none is pending:
+// agent-scope atomic loads of all of them at every poll (polling one value and
 // re-reading the others once it arrives costs a second memory round trip on
 // the dependency chain: measured 21% slower sweeps), device-scope atomic adds
 // of zero after kSpinAtomic polls.  Gives up after the spin limit, or at once
@@ -162,9 +162,10 @@ __device__ __forceinline__ void wait_read(double* v5, double out[5], int* stalls
 
 // The mixed form publishes prim(U + delU) as five FP32 values in three 64-bit
 // words (its consumers evaluate their fluxes in FP32 anyway): lo/hi halves
-// p0/p1, p2/p3, p4/0.  A word with a zero high half never matches kPending,
-// and the other two would need the FP32 NaN 0x7FF4DEAD, which arithmetic does
-// not produce.
+// p0/p1, p2/p3, p4/0.  A word with a zero high half never matches kPending;
+// the other two would need a high half of 0x7FF4DEAD, a NaN payload that
+// arithmetic does not produce (it yields the default NaN, or passes on the
+// payload of a NaN operand, and no operand here carries that one).
 __device__ __forceinline__ unsigned long long pack2(float lo, float hi) {
     return (unsigned long long)__float_as_uint(lo) | ((unsigned long long)__float_as_uint(hi) << 32);
 }

@@ -978,6 +978,20 @@ int bind_impl(mfcfd_ctx* c, const mfcfd_geometry* gm, const int* inst_of, int n_
             case 6: lso.push_back(d); break;
         }
     }
+    // the general class grouped by status -- wall, outer, then the interior
+    // points the spill altered -- within windows of kGroupWindow points, each
+    // group in device order: a wave's points then take the same flux path
+    // (octant or half-range lists) instead of both, while every XCD's
+    // contiguous share of the list keeps the same mix of classes (grouping
+    // the whole list gave the octant-heavy wall points to a few XCDs)
+    {
+        constexpr size_t kGroupWindow = 4096;
+        const auto rank = [&](int d) { return status[d] == 1 ? 0 : status[d] == 2 ? 1 : 2; };
+        for (size_t w0 = 0; w0 < lfw.size(); w0 += kGroupWindow) {
+            const size_t w1 = std::min(lfw.size(), w0 + kGroupWindow);
+            std::stable_sort(lfw.begin() + w0, lfw.begin() + w1, [&](int a, int b) { return rank(a) < rank(b); });
+        }
+    }
     c->n_fused_interior_pts = (int)lfi.size();
     c->n_fused_general_pts = (int)lfw.size();
     // the reference visits wall points, then interior points, each in host order

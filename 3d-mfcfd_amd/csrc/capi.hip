@@ -162,6 +162,7 @@ struct mfcfd_ctx {
         DevBuf<int> seq, old_pts, old_src, wait_src;
         DevBuf<long long> take_ptr;
         DevBuf<unsigned> take;
+        DevBuf<unsigned char> late;
         DevBuf<double> wgt;
         std::vector<int> level_ptr;   // host copy of the level cuts of seq
         int count = 0;
@@ -172,6 +173,7 @@ struct mfcfd_ctx {
             wait_src.release();
             take_ptr.release();
             take.release();
+            late.release();
             wgt.release();
             level_ptr.clear();
             count = 0;
@@ -423,6 +425,7 @@ SweepAsync sweep_args(mfcfd_ctx* c, mfcfd_ctx::DevSweepDir& d) {
     a.count = d.count;
     a.take_ptr = d.take_ptr.p;
     a.take = d.take.p;
+    a.late = d.late.p;
     a.wgt = d.wgt.p;
     a.old_pts = d.old_pts.p;
     a.n_old = (int)d.old_pts.n;
@@ -543,6 +546,9 @@ int build_plan(mfcfd_ctx* c, int form) {
             return r;
         if ((r = upload(d.take_ptr, pd.take_ptr.data(), pd.take_ptr.size(), st))) return r;
         if ((r = upload(d.take, pd.take.empty() ? &none_u : pd.take.data(), std::max<size_t>(pd.take.size(), 1), st)))
+            return r;
+        const unsigned char none_c = 0;
+        if ((r = upload(d.late, pd.late.empty() ? &none_c : pd.late.data(), std::max<size_t>(pd.late.size(), 1), st)))
             return r;
         if (d.wgt.alloc(std::max<size_t>(pd.take.size(), 1)) != hipSuccess)
             return fail(MFCFD_ENOMEM, "sweep weights (%zu entries)", pd.take.size());

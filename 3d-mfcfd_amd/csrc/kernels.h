@@ -58,6 +58,8 @@ struct SweepAsync {
     int count;
     const long long* take_ptr;  // [count + 1]: position t takes take[take_ptr[t] .. take_ptr[t+1])
     const unsigned* take;       // k << 4 | wait << 3 | direction (7: supersonic copy) of every taken entry
+    const unsigned char* late;  // [count]: position t's last late[t] entries read the wait dependency
+                                // that executes last (at most 4; 0: none set apart)
     double* wgt;                // [entries] least-squares weight of each entry (set at bind)
     const int* old_pts;         // points some entry reads as old: pold is taken for them
     int n_old;

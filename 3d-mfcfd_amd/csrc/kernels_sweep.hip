@@ -40,7 +40,12 @@
 //                  longer pending for a wait entry, read from pold for an
 //                  old one -- and Gnew from it.  The wave's sum is a
 //                  fixed-order DPP reduction that leaves the sums in lane
-//                  kTailLane; that lane publishes the point's state
+//                  kTailLane.  The entries of the dependency that executes
+//                  last (the plan puts them at the end, late[t] of them) are
+//                  held back: the wave sums the others first, then waits for
+//                  that one point on lanes 60..63 and adds their few terms
+//                  with two row shifts, so the full reduction is off the
+//                  dependency chain.  Lane kTailLane publishes the point's state
 //                  prim(U_i + delU_i), computed once here instead of once
 //                  per entry that reads it, then stores its delU.  The
 //                  mixed form publishes that state as five FP32 values in
@@ -410,8 +415,36 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
                 for (int r = 0; r < 5; ++r)
                     own[r] = FORWARD ? s.flux_res[5 * (long long)i + r] : s.delUp[5 * (long long)i + r];
             }
+            // The entries of the wait dependency that executes last (it
+            // usually publishes last, well after the others) sit at the end of
+            // the list, at most kMaxLate of them, one per lane from lane
+            // 64 - n_late.  Their Gold is computed now, before any wait; the
+            // other entries are summed over the wave first, and these then
+            // only add their own terms.  The mixed form only: in FP64 the
+            // second flux evaluation this costs every position (a whole
+            // wave's worth of instructions for a few lanes) loads the SIMDs
+            // more than the shorter chain gains (measured: 35.9 -> 38.1 ms
+            // per FP64 sweep, 30.7 -> 29.2 ms per mixed one).
+            const int n_late = sizeof(T) == 4 ? __builtin_amdgcn_readfirstlane((int)a.late[t]) : 0;
+            const long long tl = te - n_late;
+            const int lj = lane - (64 - n_late);   // >= 0 on a late lane
+            T Gold_l[5] = {T(0), T(0), T(0), T(0), T(0)};
+            double w_l = 0.0;
+            long long k_l = 0;
+            int d_l = 0;
+            if (lj >= 0) {
+                const unsigned packed = a.take[tl + lj];
+                k_l = (long long)(packed >> kTakeShift);
+                d_l = (int)(packed & 7u);
+                w_l = a.wgt[tl + lj];
+                T prim_old[5];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) prim_old[r] = T(s.prim[5 * k_l + r]);
+                flux_eval<T, true>(dir_family(cls, d_l), d_l >> 1, (d_l & 1) ? -1 : 1, ft, prim_old, T(fp.pi),
+                                   Gold_l);
+            }
             double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            for (long long e = tb + lane; e < te; e += 64) {
+            for (long long e = tb + lane; e < tl; e += 64) {
                 const unsigned packed = a.take[e];
                 const long long k = (long long)(packed >> kTakeShift);
                 const int d = (int)(packed & 7u);
@@ -438,15 +471,43 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_async(DevGeom g, DevState
 #pragma unroll
                     for (int r = 0; r < 5; ++r) pt[r] = T(a.pold[5 * k + r]);
                 }
-                // from here to the publication the wave is on the dependency
-                // chain: its SIMD issues for it before a wave still working
-                // ahead of its waits
-                __builtin_amdgcn_s_setprio(3);
+                // with no late entries set apart, the wave is on the dependency
+                // chain from here to the publication: its SIMD issues for it
+                // before a wave still working ahead of its waits
+                if (n_late == 0) __builtin_amdgcn_s_setprio(3);
                 flux_eval<T, true>(dir_family(cls, d), d >> 1, (d & 1) ? -1 : 1, ft, pt, T(fp.pi), Gnew);
 #pragma unroll
                 for (int r = 0; r < 5; ++r) acc[r] = acc[r] + (double)(Gnew[r] - Gold[r]) * wgt;
             }
             wave_sum5(acc);
+            if (n_late > 0) {
+                // the late entries: their terms, summed into lane kTailLane by
+                // two row shifts over lanes 60..63, then added to the others'
+                double c[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                if (lj >= 0) {
+                    T pt[5], Gnew[5];
+                    if constexpr (sizeof(T) == 4) {
+                        wait_read_packed(a.pub + 5 * k_l, pt, a.stalls, a.zero);
+                    } else {
+                        double pk[5];
+                        wait_read(a.pub + 5 * k_l, pk, a.stalls, a.zero);
+#pragma unroll
+                        for (int r = 0; r < 5; ++r) pt[r] = T(pk[r]);
+                    }
+                    // on the dependency chain from here to the publication
+                    __builtin_amdgcn_s_setprio(3);
+                    flux_eval<T, true>(dir_family(cls, d_l), d_l >> 1, (d_l & 1) ? -1 : 1, ft, pt, T(fp.pi),
+                                       Gnew);
+#pragma unroll
+                    for (int r = 0; r < 5; ++r) c[r] = (double)(Gnew[r] - Gold_l[r]) * w_l;
+                }
+#pragma unroll
+                for (int r = 0; r < 5; ++r) c[r] = c[r] + dpp_move<0x111, 0xf>(c[r]);
+#pragma unroll
+                for (int r = 0; r < 5; ++r) c[r] = c[r] + dpp_move<0x112, 0xf>(c[r]);
+#pragma unroll
+                for (int r = 0; r < 5; ++r) acc[r] = acc[r] + c[r];
+            }
             if (lane == kTailLane) {
                 // wall terms are doubled (implicit_LdelU_forward_sweep_wall.f90:405);
                 // the product with 1 or 2 is exact

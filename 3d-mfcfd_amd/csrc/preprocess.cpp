@@ -192,6 +192,7 @@ int build_dir(const SweepPlanInputs& in, bool forward, SweepDirPlan& out) {
     for (int u = 0; u < m; ++u) order[fill[level[visit[u]] - 1]++] = u;
     out.seq.resize(m);
     out.take_ptr.assign(m + 1, 0);
+    out.late.assign(m, 0);
     out.take.clear();
     out.take.reserve(vtake.size());
     std::vector<int> exec_at(n, -1);
@@ -204,7 +205,27 @@ int build_dir(const SweepPlanInputs& in, bool forward, SweepDirPlan& out) {
         // by neighbour: a neighbour sits in up to one list per axis (more
         // with the spill), and its entries then sit on adjacent lanes, of
         // which the first reads the neighbour's state for all of them
-        std::sort(out.take.begin() + out.take_ptr[e], out.take.end());
+        const auto b = out.take.begin() + out.take_ptr[e];
+        std::sort(b, out.take.end());
+        // The wait dependency that executes last usually publishes last, well
+        // after the others (on a 500k-point cloud: for 95% of positions, a
+        // median 2.7 us after the one before it).  Its entries go last, and
+        // the sweep kernel sums the others before it waits for this one.
+        int k_last = -1, e_last = -1, n_last = 0;
+        for (auto it = b; it != out.take.end(); ++it) {
+            const unsigned x = *it;
+            if (!(x & kTakeWait) || (x & 7u) == kTakeCopy) continue;
+            const int k = (int)(x >> kTakeShift);
+            if (exec_at[k] > e_last) e_last = exec_at[k], k_last = k;
+        }
+        if (k_last >= 0) {
+            const auto is_last = [&](unsigned x) {
+                return (x & kTakeWait) && (x & 7u) != kTakeCopy && (int)(x >> kTakeShift) == k_last;
+            };
+            std::stable_partition(b, out.take.end(), [&](unsigned x) { return !is_last(x); });
+            n_last = (int)std::count_if(b, out.take.end(), is_last);
+        }
+        out.late[e] = (unsigned char)(n_last <= kMaxLate ? n_last : 0);
     }
     // every wait entry must read a point that executes earlier
     for (int e = 0; e < m; ++e)

@@ -40,12 +40,16 @@ struct SweepDirPlan {
     std::vector<int> level_ptr;       // level l covers seq[level_ptr[l] .. level_ptr[l+1])
     std::vector<long long> take_ptr;  // position u takes take[take_ptr[u] .. take_ptr[u+1])
     std::vector<unsigned> take;       // k << 4 | wait << 3 | direction (kTakeCopy: copy entry)
+    std::vector<unsigned char> late;  // per position: its last entries, which read the wait
+                                      // dependency that executes last (0..kMaxLate; 0: none)
     std::vector<int> old_pts;         // points some regular entry reads as old
     std::vector<int> old_src;         // copy sources read as old
     std::vector<int> wait_src;        // copy sources read as this sweep's value
 };
 
 constexpr unsigned kTakeWait = 8u;
+// at most this many entries of a position are set apart as late (one lane each)
+constexpr int kMaxLate = 4;
 constexpr unsigned kTakeCopy = 7u;
 constexpr int kTakeShift = 4;
 

@@ -34,10 +34,13 @@ void san_aliasing(int n, const int* nbhs, const int* conn, int* alias, int* pwa)
 // Builds the plan of the canonical (backup = 0) or alternative (1) driver.
 // Outputs, per direction (0 forward, 1 backward): seq (<= n), take_ptr
 // (<= n + 1) and take (<= cap entries); sizes[3 * dir + {0,1,2}] receive
-// their lengths.  Returns build()'s code, or -2 when cap is too small.
-int san_sweep_plan(int n, const int* status, const int* nbhs, const int* conn, const int* cnt, const int* lists,
-                   const int* alias, const int* pwa, const double* x, const double* y, const double* z, int backup,
-                   long long cap, int* seq, long long* take_ptr, unsigned* take, long long* sizes) {
+// their lengths; late (when not null, <= n per direction) each position's
+// count of trailing entries set apart as late.  Returns build()'s code, or
+// -2 when cap is too small.
+int san_sweep_plan_late(int n, const int* status, const int* nbhs, const int* conn, const int* cnt,
+                        const int* lists, const int* alias, const int* pwa, const double* x, const double* y,
+                        const double* z, int backup, long long cap, int* seq, long long* take_ptr, unsigned* take,
+                        long long* sizes, unsigned char* late) {
     std::vector<int> dcnt(6LL * n, 0), conn0(27LL * n, -1), lst;
     std::vector<long long> off(6LL * n, 0);
     for (int i = 0; i < n; ++i)
@@ -100,10 +103,19 @@ int san_sweep_plan(int n, const int* status, const int* nbhs, const int* conn, c
         for (std::size_t u = 0; u < pd.seq.size(); ++u) seq[(long long)d * n + u] = pd.seq[u];
         for (std::size_t u = 0; u < pd.take_ptr.size(); ++u) take_ptr[(long long)d * (n + 1) + u] = pd.take_ptr[u];
         for (std::size_t u = 0; u < pd.take.size(); ++u) take[d * cap + u] = pd.take[u];
+        if (late)
+            for (std::size_t u = 0; u < pd.late.size(); ++u) late[(long long)d * n + u] = pd.late[u];
         sizes[3 * d] = (long long)pd.seq.size();
         sizes[3 * d + 1] = (long long)pd.take_ptr.size();
         sizes[3 * d + 2] = (long long)pd.take.size();
     }
     return 0;
+}
+
+int san_sweep_plan(int n, const int* status, const int* nbhs, const int* conn, const int* cnt, const int* lists,
+                   const int* alias, const int* pwa, const double* x, const double* y, const double* z, int backup,
+                   long long cap, int* seq, long long* take_ptr, unsigned* take, long long* sizes) {
+    return san_sweep_plan_late(n, status, nbhs, conn, cnt, lists, alias, pwa, x, y, z, backup, cap, seq, take_ptr,
+                               take, sizes, nullptr);
 }
 }

@@ -43,6 +43,9 @@ _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 
 
+MAX_LATE = 4
+
+
 def lib():
     if not os.path.exists(LIB):
         if shutil.which("make") is None:
@@ -54,6 +57,9 @@ def lib():
     L.san_sweep_plan.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, C.c_int,
                                  C.c_longlong, _ip, _lp, _up, _lp]
     L.san_sweep_plan.restype = C.c_int
+    L.san_sweep_plan_late.argtypes = L.san_sweep_plan.argtypes + [
+        np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")]
+    L.san_sweep_plan_late.restype = C.c_int
     return L
 
 
@@ -176,3 +182,56 @@ def test_a_consumer_ahead_of_its_producer_deadlocks():
     where = {v: i for i, v in enumerate(order)}
     planted = [sorted(where[v] for v in deps[o]) for o in order]
     assert simulate(planted, 1, 1) < P
+
+
+@pytest.mark.parametrize("name,backup", [("implicit_morton", False), ("implicit_backup", True),
+                                         ("implicit_supersonic", False)])
+def test_late_entries_read_the_dependency_that_executes_last(name, backup):
+    """Each position's last late[u] entries are wait entries of one point, the
+    wait dependency that executes last; no other entry of the position reads
+    it; a position sets none apart only when it has no regular wait entry or
+    that point fills more than MAX_LATE entries.  The sweep kernel sums the
+    other entries before it waits for these."""
+    cl, _ = load_fixture(name)
+    n = cl.n
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
+    i = lambda a: np.ascontiguousarray(a, dtype=np.int32).ravel()
+    cnt, lists = solver.split_stencils(cl)
+    alias, pwa = solver.alias_order(cl)
+    cap = 200 * n
+    seq = np.zeros(2 * n, dtype=np.int32)
+    take_ptr = np.zeros(2 * (n + 1), dtype=np.int64)
+    take = np.zeros(2 * cap, dtype=np.uint32)
+    sizes = np.zeros(6, dtype=np.int64)
+    late = np.zeros(2 * n, dtype=np.uint8)
+    rc = lib().san_sweep_plan_late(n, i(cl.status), i(cl.nbhs), i(cl.conn), i(cnt), i(lists), i(alias), i(pwa),
+                                   f(cl.x), f(cl.y), f(cl.z), int(backup), cap, seq, take_ptr, take, sizes, late)
+    assert rc == 0
+    set_apart = 0
+    for dr in range(2):
+        ns, ntp, nt = (int(v) for v in sizes[3 * dr:3 * dr + 3])
+        s_ = seq[dr * n:dr * n + ns]
+        tp = take_ptr[dr * (n + 1):dr * (n + 1) + ntp]
+        tk = take[dr * cap:dr * cap + nt].astype(np.int64)
+        lt = late[dr * n:dr * n + ns]
+        pos = np.full(n, -1, dtype=np.int64)
+        pos[s_] = np.arange(ns)
+        for u in range(ns):
+            e = tk[tp[u]:tp[u + 1]]
+            regular_wait = ((e & TAKE_WAIT) != 0) & ((e & 7) != 7)
+            nl = int(lt[u])
+            if not regular_wait.any():
+                assert nl == 0
+                continue
+            k = e >> TAKE_SHIFT
+            k_last = k[regular_wait][np.argmax(pos[k[regular_wait]])]
+            reads_last = regular_wait & (k == k_last)
+            if nl == 0:
+                assert reads_last.sum() > MAX_LATE, u
+                continue
+            set_apart += 1
+            assert 1 <= nl <= MAX_LATE
+            assert reads_last.sum() == nl
+            assert reads_last[len(e) - nl:].all()
+    assert set_apart > 0
+

@@ -97,10 +97,13 @@ namespace mfcfd {
 namespace {
 
 // waves per workgroup: one point per wave, and one ticket counter atomic per
-// workgroup and round for kSweepWaves consecutive positions (same-address
-// device-scope atomics serialise: ~12 ns each, which caps single-ticket
-// rounds at ~80 points per microsecond)
-constexpr int kSweepWaves = 4;
+// workgroup and round for kSweepWaves consecutive positions.  Same-address
+// device-scope atomics serialise (~12 ns each), so a ticket per 4 positions
+// capped the sweep near 330 positions per microsecond, close to what the 8M
+// sweeps need (220 FP64, 270 mixed): with 8 per ticket the FP64 sweep took
+// 35.3 -> 33.6 ms and the mixed one 29.2 -> 26.3 ms.  8 waves of a workgroup
+// still fit the FP64 kernel's 2 waves per SIMD.
+constexpr int kSweepWaves = 8;
 constexpr int kSweepBlock = 64 * kSweepWaves;
 // about a second of polling; legitimate waits end within milliseconds
 constexpr int kSpinLimit = 1 << 20;

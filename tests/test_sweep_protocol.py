@@ -1,7 +1,7 @@
 """The sync-free sweep's progress protocol, modelled on the CPU.
 
-k_sweep_async (kernels_sweep.hip) runs a resident grid of workgroups of four
-waves.  Each round, thread 0 of a workgroup takes four consecutive execution
+k_sweep_async (kernels_sweep.hip) runs a resident grid of workgroups of eight
+waves.  Each round, thread 0 of a workgroup takes eight consecutive execution
 positions with one atomic add on a ticket counter, a workgroup barrier
 follows, wave w works on position base + w -- it waits, for every wait entry
 of its take list, until the entry's point has published -- then publishes,
@@ -35,7 +35,7 @@ solver = PKG.solver
 SAN = os.path.join(ROOT, "tests", "sanitize")
 LIB = os.path.join(SAN, "_build", "libpreprocess_plain.so")
 TAKE_WAIT, TAKE_SHIFT = 8, 4
-WAVES = 4
+WAVES = 8
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

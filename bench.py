@@ -11,7 +11,7 @@ timed region; its mixed-precision twin is reported beside it.
 Only finite states are timed.  At CFL 50 the reference's own canonical
 implicit pairing diverges (tests/golden/cfl50_20k_history.npz: the far
 field grows until the pressures turn non-finite), and on the 8M cloud the
-FP64 run turns non-finite at iteration 14, the mixed one at iteration 9.  So
+FP64 run turns non-finite at iteration 14, the mixed one at iteration 9-10.  So
 the state after W warmup iterations is saved, and the K timed iterations run
 in blocks of at most --block iterations, each block starting from that saved
 state (restored by host copies outside the timed region): with the

@@ -105,6 +105,8 @@ namespace {
 // still fit the FP64 kernel's 2 waves per SIMD.
 constexpr int kSweepWaves = 8;
 constexpr int kSweepBlock = 64 * kSweepWaves;
+// the late entries sit on lanes 64 - kMaxLate .. 63, which two row shifts sum
+static_assert(kMaxLate >= 1 && kMaxLate <= 4, "late lanes must fit the last four lanes of a row");
 // about a second of polling; legitimate waits end within milliseconds
 constexpr int kSpinLimit = 1 << 20;
 // polls by atomic load before the atomic-add fallback
